@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Projected points/s of the MLoD projection hot path on one MI355X.
+
+Workload (default): the 8M-point precision form — a superquadric (exponents 0.4/0.4, axes
+1/0.7/0.5) with multi-octave sinusoidal displacement (amplitude 0.01) and Gaussian noise
+(sigma 0.001), 8M off-surface queries at uniform offsets +-0.02 diag, multi-Gaussian kernel
+with 6 sigmas (sigma_1 = 0.01 R, factor 2, scaled by the bbox radius R), cutoff 2,
+minDepth 0, octree depth <= 12 with one point per leaf.
+
+A "step" is one projection step of all M queries (APSS::project(M, ..., 1, ...), the
+reference harness's unit, scene.cpp:929-933): one kernel launch on device-resident queries,
+projected in place.  value = M * steps / time (projected points/s), timed with HIP events on
+the launch stream.  The same K steps are run in the plain FP32 and plain FP64 forms; the
+FP64 run is the error reference (max / RMS |dp| / diag).  The CPU baseline is the
+reference's own OpenMP CPU path (its CPUUtils.cpp compiled into oracle/_ref) on a bounded
+query sample on this host.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--scale 1.0]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def weight_cost(kern):
+    """(flops, transcendentals) of one weight evaluation."""
+    t = kern["type"]
+    if t == 2:
+        s = len(kern["sigmas"])
+        return 3 * s, s
+    if t == 0:
+        return 3, 1
+    return 6, 1
+
+
+def algorithmic_work(counts, m, kern):
+    """FP32 flops (FMA = 2) and transcendentals of one step over m queries, from the
+    traversal counters: each far node, leaf point and near node's own share costs the
+    offset and squared distance (8), the weight, f = m w (1) and 9 moment FMAs (18); each
+    tested child its distance (8), sqrt, far test (2) and blend factor (10 flops, 2
+    transcendentals); each query the sphere fit (70)."""
+    wf, wt = weight_cost(kern)
+    contrib = counts["far"] + counts["leaf_points"] + counts["near"]
+    flops = contrib * (27 + wf) + counts["tested"] * 20 + 70 * m
+    trans = contrib * wt + counts["tested"] * 3
+    return float(flops), float(trans)
+
+
+def cpu_baseline(w, kern, cutoff, budget_s=15.0):
+    """The reference's CPU projection (APSSOctree::projectCPU with OpenMP over queries,
+    scene.cpp:982-986), REF_CPU semantics, one step per query, on a bounded sample."""
+    from oracle import bind
+    if not bind.have_ref():
+        return None
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    t0 = time.time()
+    tree = bind.RefTree(w["points"], w["normals"])
+    build_s = time.time() - t0
+    k = bind.make_kernel(**kern)
+    prm = bind.make_params(cutoff=cutoff, semantics="ref_cpu")
+    q = w["queries"]
+    n0 = min(len(q), 4000)
+    t0 = time.time()
+    tree.project(q[:n0], k, prm, n_iter=1, nthreads=cores)
+    rate = n0 / max(time.time() - t0, 1e-6)
+    ns = int(min(len(q), max(n0, rate * budget_s)))
+    t0 = time.time()
+    tree.project(q[:ns], k, prm, n_iter=1, nthreads=cores)
+    dt = time.time() - t0
+    return {"value": ns / dt, "unit": "projected pts/s", "cores": cores, "kind": "reference",
+            "sample": f"first {ns} of {len(q)} queries, 1 step each, reference APSSOctree::projectCPU (REF_CPU) "
+                      f"with OpenMP on {cores} host threads; reference octree build {build_s:.1f} s untimed"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="superquadric8m")
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--precision", default="fp32c", choices=["fp32", "fp32c", "fp64"])
+    ap.add_argument("--semantics", default="ref_gpu", choices=["ref_gpu", "ref_cpu"])
+    ap.add_argument("--no-forms", action="store_true", help="skip the plain FP32 / FP64 comparison runs")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only.  The projection path would shard by "
+              "replicating the octree and splitting the Morton-ordered queries across GPUs; that is not "
+              "built here.  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from mlodsurfaces_amd import apss as A
+    from mlodsurfaces_amd import datasets as ds
+    from mlodsurfaces_amd import hiprt
+
+    t0 = time.time()
+    w = ds.make_workload(args.workload, scale=args.scale)
+    P, N, Q = w["points"], w["normals"], w["queries"]
+    n, m = len(P), len(Q)
+    lo, hi = P.min(0).astype(np.float64), P.max(0).astype(np.float64)
+    diag = float(np.linalg.norm(hi - lo))
+    log(f"workload {w['description']} generated in {time.time() - t0:.1f}s")
+
+    stream = hiprt.Stream()
+    dP, dN, dQ = hiprt.DeviceBuffer(P), hiprt.DeviceBuffer(N), hiprt.DeviceBuffer(Q)
+    hiprt.device_synchronize()
+    tb0, tb1 = hiprt.Event(), hiprt.Event()
+    tb0.record(stream)
+    tree = A.APSSOctree().build_device(dP.ptr.value, dN.ptr.value, n, stream=stream.ptr)
+    tb1.record(stream)
+    tb1.synchronize()
+    build_ms = tb0.elapsed_ms(tb1)
+    info = tree.info()
+    log(f"octree: {info['n_nodes']} nodes, depth {info['depth']}, ties {info['ties']}, build {build_ms:.1f} ms, "
+        f"{info['device_bytes'] / 2**20:.0f} MiB")
+
+    K = A.Kernel(**w["kernel"])
+    forms = [args.precision] + ([] if args.no_forms else [f for f in ("fp32", "fp64") if f != args.precision])
+    if "fp64" in forms:
+        forms = [f for f in forms if f != "fp64"] + ["fp64"]
+    results = {}
+    counts = None
+    for form in forms:
+        ap_ = A.Fastapss(tree, K)
+        ap_.setScalingProtectionSphere(w["cutoff"])
+        ap_.setMinDepth(0)
+        ap_.set_semantics(args.semantics)
+        ap_.set_precision(form)
+        ap_.copyPointsToGPU_device(dQ.ptr.value, m, stream=stream.ptr)
+        if counts is None:
+            counts = ap_.count_visits(stream=stream.ptr)
+            log(f"counters (one step): {counts}")
+        for _ in range(args.warmup):
+            ap_.project(m, None, None, 1, K, stream=stream.ptr)
+        ap_.copyPointsToGPU_device(dQ.ptr.value, m, stream=stream.ptr)
+        stream.synchronize()
+        hiprt.device_synchronize()
+        ev = [hiprt.Event() for _ in range(args.steps + 1)]
+        wall0 = time.perf_counter()
+        ev[0].record(stream)
+        for s in range(args.steps):
+            ap_.project(m, None, None, 1, K, stream=stream.ptr)
+            ev[s + 1].record(stream)
+        ev[-1].synchronize()
+        hiprt.device_synchronize()
+        wall = time.perf_counter() - wall0
+        launch_ms = [ev[s].elapsed_ms(ev[s + 1]) for s in range(args.steps)]
+        total_ms = ev[0].elapsed_ms(ev[-1])
+        pts, nrm = ap_.download(m, stream=stream.ptr)
+        results[form] = dict(total_ms=total_ms, wall_s=wall, launch_ms=launch_ms, pts=pts, nrm=nrm,
+                             pts_per_s=m * args.steps / (total_ms / 1e3))
+        log(f"{form}: {args.steps} steps in {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms), "
+            f"{results[form]['pts_per_s'] / 1e6:.1f} M projected pts/s, mean launch {np.mean(launch_ms):.2f} ms")
+
+    ref64 = results.get("fp64")
+    errs = {}
+    for form, r in results.items():
+        if ref64 is not None and form != "fp64":
+            e = np.linalg.norm(r["pts"].astype(np.float64) - ref64["pts"].astype(np.float64), axis=1) / diag
+            ok = np.isfinite(e)
+            errs[form] = dict(max=float(e[ok].max()), rms=float(np.sqrt(np.mean(e[ok] ** 2))), nonfinite=int((~ok).sum()))
+            log(f"{form} vs fp64 after {args.steps} steps: max {errs[form]['max']:.3e} rms {errs[form]['rms']:.3e} diag")
+
+    head = results[args.precision]
+    mean_launch_s = float(np.mean(head["launch_ms"])) / 1e3
+    flops, trans = algorithmic_work(counts, m, w["kernel"])
+    achieved = (flops + 8.0 * trans) / mean_launch_s / 1e12
+    comp_bytes = 64.0 * info["n_nodes"] + 32.0 * n + 48.0 * m
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "algorithmic_flops_per_launch": flops, "transcendentals_per_launch": trans,
+                "compulsory_bytes_per_launch": comp_bytes,
+                "compulsory_hbm_frac": comp_bytes / mean_launch_s / 1e9 / PEAK_HBM_GBS,
+                "mean_launch_ms": mean_launch_s * 1e3}
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(w, w["kernel"], w["cutoff"], args.cpu_budget)
+            if cpu:
+                log(f"cpu baseline: {cpu['value'] / 1e6:.3f} M pts/s on {cpu['cores']} threads")
+        except Exception as ex:  # the baseline is reported, never the measured product
+            log(f"cpu baseline failed: {ex}")
+    out = {
+        "metric": "projected pts/s (8M-pt MLoD, <=1e-6*diag vs FP64) vs plain; serve p50/p99 ms",
+        "value": head["pts_per_s"],
+        "unit": "projected pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["total_ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32c": "fp32-compensated+fp64-fit", "fp32": "fp32", "fp64": "fp64"}[args.precision],
+        "data": "synthetic (seeded superquadric cloud and offset queries)",
+        "config": {"workload": args.workload if args.scale == 1.0 else f"{args.workload}@{args.scale}",
+                   "points": n, "queries": m, "kernel": "multi-Gaussian S=%d" % len(w["kernel"].get("sigmas", [])),
+                   "cutoff": w["cutoff"], "min_depth": 0, "semantics": args.semantics, "precision": args.precision,
+                   "octree_nodes": info["n_nodes"], "octree_depth": info["depth"], "build_ms": build_ms,
+                   "parallelism": "single GPU"},
+        "error_vs_fp64": errs,
+        "forms": {f: {"pts_per_s": r["pts_per_s"], "ms_per_step": r["total_ms"] / args.steps} for f, r in results.items()},
+        "counters_per_step": counts,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/* mlod.h — C ABI of the MI355X-native MLoD / APSS projection library (libmlod.so).
+ *
+ * The library replaces the reference's projection plugin interface, `class APSS`
+ * (CorentinMercier/MlodSurfaces, sources/apss/GPUUtils.h:22-67), implemented there by
+ * `Fastapss` (sources/apss/Fastapss.h:32-51) and `GlobalAPSS` (sources/apss/GlobalAPSS.h:19-43),
+ * together with the octree it projects on, `APSSOctree` (sources/apss/CPUUtils.h:166-229).
+ * Each entry point below names the reference member it stands in for.
+ *
+ * Conventions: plain pointers and sizes, no C++ or torch types.  Every call returns
+ * MLOD_OK (0) or a negative status; mlod_last_error() gives the message of the calling
+ * thread's last failure (the reference instead calls exit() from gpuErrchk,
+ * GPUUtils.cu:32-41).  `stream` is a hipStream_t passed as void* (NULL = the default
+ * stream).  One handle must not be used from two threads at once.
+ */
+#ifndef MLOD_H
+#define MLOD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mlod_tree mlod_tree;
+
+enum {
+  MLOD_OK = 0,
+  MLOD_ERR_INVALID = -1,     /* bad argument */
+  MLOD_ERR_HIP = -2,         /* HIP runtime error */
+  MLOD_ERR_OOM = -3,         /* device allocation failed */
+  MLOD_ERR_UNSUPPORTED = -4, /* option not implemented on the GPU path */
+  MLOD_ERR_STATE = -5        /* call out of order (e.g. project before queries) */
+};
+
+/* Kernel::KernelType numbering (sources/apss/Kernel.h:37), as persisted in .params files. */
+enum { MLOD_KERNEL_GAUSSIAN = 0, MLOD_KERNEL_SINGULAR = 1, MLOD_KERNEL_GAUSSIAN_MULTIPLE = 2 };
+
+/* Weight kernel (Kernel fields, Kernel.h:38-46).  sigma / sigmas are in the point-cloud
+ * frame, i.e. already multiplied by the octree bbox radius as scene::setNewKernel does
+ * (scene.cpp:1025-1033); constant_shift is not scaled. */
+#define MLOD_MAX_SIGMAS 32
+typedef struct {
+  int32_t type;
+  float sigma;          /* GAUSSIAN */
+  float exponent;       /* SINGULAR ("rational") */
+  float constant_shift; /* GAUSSIAN / SINGULAR */
+  int32_t n_sigmas;     /* GAUSSIAN_MULTIPLE */
+  float sigmas[MLOD_MAX_SIGMAS];
+} mlod_kernel;
+
+/* APSSOctree::build options (CPUUtils.h:176-180). */
+typedef struct {
+  uint32_t max_depth;        /* <= 12 (MAX_DEPTH, CPUUtils.h:20) */
+  uint32_t max_pts_per_leaf; /* the GPU build supports 1, the reference default */
+  int32_t tie_policy;        /* exact-midpoint ties: 0 = low child, 1 = high child
+                                (the reference draws rand(), CPUUtils.cpp:299-308) */
+} mlod_build_opts;
+
+/* Which of the reference's two projection functions to compute (they differ,
+ * Fastapss.cu vs CPUUtils.cpp: leaf points, parent share, plane threshold). */
+enum { MLOD_SEM_REF_GPU = 0, MLOD_SEM_REF_CPU = 1 };
+
+/* Arithmetic form of the projection. */
+enum {
+  MLOD_PREC_FP32 = 0,      /* plain: global-frame FP32 moments and FP32 fit (the reference's form) */
+  MLOD_PREC_FP32C = 1,     /* precision: node-centred moments, query-recentred compensated FP32
+                              accumulation, FP64 sphere fit */
+  MLOD_PREC_FP64 = 2       /* plain FP64 everywhere */
+};
+
+/* Neighbourhood: MLoD octree (Fastapss), all points (GlobalAPSS), k nearest or ball
+ * (APSSOctree::knnMode / ballMode, CPUUtils.h:191-192). */
+enum { MLOD_MODE_MLOD = 0, MLOD_MODE_GLOBAL = 1, MLOD_MODE_KNN = 2, MLOD_MODE_BALL = 3 };
+
+typedef struct {
+  uint32_t n_points;
+  uint32_t n_nodes;
+  uint32_t depth;           /* deepest node depth (APSSOctree computeDepth) */
+  uint32_t level_offset[14];/* BFS index of the first node of each depth; [depth+1] = n_nodes */
+  float bbox[6];            /* root bb (xyz), BB (xyz) */
+  float radius;             /* root bbox radius: default step bound and sigma scale */
+  uint64_t ties;            /* exact-midpoint ties met by the build */
+  uint64_t device_bytes;    /* HBM held by the tree (nodes, points, moments) */
+} mlod_tree_info;
+
+const char* mlod_last_error(void);
+
+/* APSSOctree::build + Fastapss::copyToGPU (CPUUtils.cpp:540-569, Fastapss.cu:457-484):
+ * builds the octree, point areas and node moments on the GPU from host arrays
+ * (n x 3 floats each).  opts may be NULL (max_depth 12, 1 point per leaf, ties low). */
+int mlod_tree_create(const float* xyz, const float* nrm, uint32_t n, const mlod_build_opts* opts, void* stream,
+                     mlod_tree** out);
+/* Same from device-resident arrays. */
+int mlod_tree_create_device(const float* d_xyz, const float* d_nrm, uint32_t n, const mlod_build_opts* opts,
+                            void* stream, mlod_tree** out);
+/* Fastapss::eraseFromGPU + APSS destructor (Fastapss.cu:486-511, GPUUtils.h:24). */
+int mlod_tree_destroy(mlod_tree* t);
+int mlod_tree_get_info(const mlod_tree* t, mlod_tree_info* out);
+
+/* APSS::updateKernel (GPUUtils.cu:466-472). */
+int mlod_set_kernel(mlod_tree* t, const mlod_kernel* k);
+/* APSS::setScalingProtectionSphere / setMinDepth (GPUUtils.h:40-43) plus the semantics
+ * and arithmetic form. */
+int mlod_set_params(mlod_tree* t, float cutoff, uint32_t min_depth, int32_t semantics, int32_t precision);
+/* APSSOctree::knnMode / ballMode (CPUUtils.cpp:674-688); ball_radius < 0 => 3 * sigmas[0]. */
+int mlod_set_mode(mlod_tree* t, int32_t mode, uint32_t knn, float ball_radius);
+
+/* APSS::copyPointsToGPU (GPUUtils.cu:406-411): uploads m query points (host, m x 3). */
+int mlod_queries_upload(mlod_tree* t, const float* xyz, uint32_t m, void* stream);
+/* Same from a device array (m x 3 floats). */
+int mlod_queries_set_device(mlod_tree* t, const float* d_xyz, uint32_t m, void* stream);
+/* APSS::erasePointsFromGPU (GPUUtils.cu:452-464). */
+int mlod_queries_erase(mlod_tree* t);
+
+/* APSS::project (GPUUtils.h:26-27, Fastapss.cu:439-455): n_iter projection steps of the
+ * first n uploaded queries, in place, so the next call continues from the projected
+ * positions.  step_max < 0 => root bbox radius.  out_pts / out_nmls (host, n x 3) are
+ * written when non-NULL (then the call waits for the stream). */
+int mlod_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, float* out_pts, float* out_nmls,
+                 void* stream);
+/* Copies the current positions / normals of the first n queries to host arrays. */
+int mlod_queries_download(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, void* stream);
+
+/* Octree dump in BFS order (the flattening of Fastapss copy(), Fastapss.cu:97-150, via
+ * getRelations, CPUUtils.cpp:213-234), for parity checks.  Any pointer may be NULL.
+ * leaf_idx receives, for each leaf in BFS order, its point indices in input order. */
+int mlod_tree_dump(const mlod_tree* t, int32_t* depth, float* bb6, float* center3, float* radius, int32_t* father,
+                   int32_t* first_child, int32_t* next_brother, int32_t* num_children, uint32_t* leaf_count,
+                   uint32_t* leaf_idx, float* areas, float* moments9);
+
+/* Traversal counters of one step over the uploaded queries (untimed): far nodes
+ * accepted, leaf points summed, child nodes tested, near (blended) nodes.
+ * out4 receives the four totals. */
+int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out4, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLOD_H */

@@ -1,0 +1,142 @@
+// mlod_internal.h — shared declarations of the HIP implementation of libmlod.so (gfx950).
+//
+// Data layout in HBM (one octree):
+//   nodes, breadth-first order (depth, then Morton prefix), so the children of a node are
+//   contiguous and the nodes of one depth form one range [level_off[d], level_off[d+1]):
+//     geo  float4 {cx, cy, cz, r}            centre and radius of the cell box
+//     hdr  uint4  {first, count|flags, A, father}
+//            first  = first child (internal) or first sorted point (leaf)
+//            count  = #children or #points, bits 27..30 depth, bit 31 leaf
+//            A      = sum of point areas (float bits, reference summation order)
+//     momP float4 x2 {ap, apn}, {an, app}     plain moments (global frame, FP32)
+//     momC float4 x2 {pbar, C}, {N, V}        centred moments: mean point, sum a n,
+//                                              sum a (p - pbar).n, sum a |p - pbar|^2
+//     momD double4 x2 + double                plain moments in FP64
+//   points, in leaf (sorted) order: pts float4 {p, a}, nrm float4 {n, 0}; order maps a
+//   sorted position to the input index.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/mlod.h"
+
+namespace mlod {
+
+constexpr int kMaxDepth = 12;          // MAX_DEPTH, sources/apss/CPUUtils.h:20
+constexpr uint32_t kLeafBit = 1u << 31;
+constexpr uint32_t kCountMask = (1u << 27) - 1;
+
+__host__ __device__ inline uint32_t hdr_count(uint32_t c) { return c & kCountMask; }
+__host__ __device__ inline uint32_t hdr_depth(uint32_t c) { return (c >> 27) & 15u; }
+__host__ __device__ inline bool hdr_leaf(uint32_t c) { return (c & kLeafBit) != 0; }
+
+// Device-side view of one tree, passed by value to kernels.
+struct TreeView {
+  const float4* geo;
+  const uint4* hdr;
+  const float4* momP;
+  const float4* momC;
+  const double4* momD;
+  const double* AD;
+  const float4* pts;
+  const float4* nrm;
+  uint32_t n, nn;
+};
+
+// Weight kernel in the form the kernels evaluate (derived on the host from mlod_kernel).
+//   multi-Gaussian  w = sum_s exp2(d2 * a[s] + b[s])   (a = -log2(e)/(2 s^2), b = -3 log2 s)
+//   Gaussian        w = exp2(d2 * a[0]) + c
+//   rational        w = (c' / (d2 + c))^e,  c' = max(c, 1e-4)
+struct KParams {
+  int32_t type;
+  int32_t ns;
+  int32_t eint;      // integer exponent (1..16) or -1
+  float c;           // constant_shift
+  float cp;          // max(c, 1e-4)
+  float e;           // exponent
+  float a[MLOD_MAX_SIGMAS];
+  float b[MLOD_MAX_SIGMAS];
+  double ad[MLOD_MAX_SIGMAS];  // FP64: -1 / (2 s^2)
+  double bd[MLOD_MAX_SIGMAS];  // FP64: 1 / s^3
+};
+
+struct QueryState {
+  float4* q;         // sorted query positions (in place across project calls)
+  float4* qn;        // normals of the last step
+  uint32_t* perm;    // sorted position -> caller's index
+  float* stage;      // m x 3 staging for un-permuted download
+  float* hstage;     // pinned host staging (m x 3 x 2)
+  uint32_t m = 0, cap = 0;
+};
+
+}  // namespace mlod
+
+struct mlod_tree {
+  int device = 0;
+  uint32_t n = 0, nn = 0, depth = 0;
+  uint32_t level_off[14] = {0};
+  float bb[3] = {0, 0, 0}, BB[3] = {0, 0, 0};
+  float radius = 0;
+  uint64_t ties = 0;
+  uint64_t bytes = 0;
+  mlod_build_opts opts{12, 1, 0};
+  // device buffers
+  float4* geo = nullptr;
+  uint4* hdr = nullptr;
+  float4* momP = nullptr;
+  float4* momC = nullptr;
+  double4* momD = nullptr;
+  double* AD = nullptr;
+  float4* pts = nullptr;
+  float4* nrm = nullptr;
+  uint32_t* order = nullptr;
+  float* box = nullptr;   // per node {bb, BB} (parity dumps)
+  float* area = nullptr;  // per point area, input order (parity dumps)
+  // parameters
+  mlod_kernel kernel{};
+  bool kernel_set = false;
+  float cutoff = 2.0f;      // m_scalingProtectionSphere default, GPUUtils.h:63
+  uint32_t min_depth = 0;   // m_minDepth default, GPUUtils.h:64
+  int32_t semantics = MLOD_SEM_REF_GPU;
+  int32_t precision = MLOD_PREC_FP32;
+  int32_t mode = MLOD_MODE_MLOD;
+  uint32_t knn = 20;
+  float ball_radius = 0.1f;
+  mlod::QueryState qs;
+  mlod::TreeView view() const {
+    return mlod::TreeView{geo, hdr, momP, momC, momD, AD, pts, nrm, n, nn};
+  }
+};
+
+namespace mlod {
+
+// error plumbing (api.cpp)
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define MLOD_HIP(expr)                                                         \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) return ::mlod::hip_fail(e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+// build.hip
+int build_tree(mlod_tree* t, const float* d_xyz, const float* d_nrm, hipStream_t s);
+int dump_tree(const mlod_tree* t, int32_t* depth, float* bb6, float* center3, float* radius, int32_t* father,
+              int32_t* first_child, int32_t* next_brother, int32_t* num_children, uint32_t* leaf_count,
+              uint32_t* leaf_idx, float* areas, float* moments9);
+
+// project.hip
+KParams make_kparams(const mlod_kernel& k);
+int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s);
+int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream_t s);
+
+// queries.hip
+int queries_set(mlod_tree* t, const float* xyz, uint32_t m, bool device_src, hipStream_t s);
+int queries_get(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, hipStream_t s);
+void queries_free(mlod_tree* t);
+
+}  // namespace mlod

@@ -1,0 +1,633 @@
+// project.hip — MLoD and global APSS projection kernels (gfx950).
+//
+// One lane per query (queries Morton-sorted by queries.hip, so a wavefront's 64 lanes walk
+// neighbouring parts of the tree).  The MLoD sum of the reference,
+//   value(leaf)  = sum over its points of w(p_i) stats_i                (Fastapss.cu:189-200)
+//   value(far)   = w(pbar) stats                                        (Fastapss.cu:214-237)
+//   value(near)  = sum_c [ b_c value(c) + (1 - b_c) share_c w(pbar) stats ] (Fastapss.cu:238-299,
+//                  CPUUtils.cpp:478-501; b_c = (1 - cos(pi t))/2, t = dP / (dP + dC))
+//   value(depth < minDepth) = sum_c value(c)                            (Fastapss.cu:303-329)
+// is linear in the stats, so it is evaluated top-down: each node carries the product m of
+// the blend factors on its path and adds m * (its own term) to one accumulator.  That needs
+// no per-depth accumulator stack (the reference keeps 12 of them per thread); the per-lane
+// traversal state is, per depth level, the first child, a mask of children still to descend,
+// the node's multiplier and its distance term, held in LDS.  A near node loops over its
+// contiguous children once: far children and leaves are summed right there, the parent's
+// (1 - b_c) share is accumulated, and only near internal children are left in the mask.
+//
+// Arithmetic forms (template FORM):
+//   0 plain FP32   — global-frame moments and the reference's FP32 sphere fit;
+//   1 FP32C        — node-centred moments, contributions recentred on the query
+//                    (sum w a (p - q), sum w a |p - q|^2, ...), Kahan-compensated FP32
+//                    accumulation, FP64 sphere fit in the query frame;
+//   2 plain FP64   — everything in double.
+// Semantics (template SEM): 0 = REF_GPU (Fastapss.cu), 1 = REF_CPU (CPUUtils.cpp), see
+// mlod.h.  Weight kernels (template KT): Kernel.h:37 numbering.
+#include <cmath>
+
+#include "mlod_internal.h"
+
+namespace mlod {
+namespace {
+
+constexpr int PBS = 128;         // lanes per workgroup
+constexpr int kLevels = kMaxDepth;  // internal nodes have depth 0..11
+
+#define LAUNCH_CHECK()                                                           \
+  do {                                                                           \
+    hipError_t e_ = hipGetLastError();                                           \
+    if (e_ != hipSuccess) return hip_fail(e_, "kernel launch", __FILE__, __LINE__); \
+  } while (0)
+
+// ----------------------------------------------------------------------------- weights
+// FP32 weights (Kernel.cpp:64-83 / GPUUtils.cu:243-257) from the squared distance.
+template <int KT>
+__device__ __forceinline__ float weight32(const KParams& k, float d2) {
+  if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
+    float w = 0.f;
+    for (int s = 0; s < k.ns; ++s) w += __builtin_amdgcn_exp2f(fmaf(d2, k.a[s], k.b[s]));
+    return w;
+  } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN) {
+    return __builtin_amdgcn_exp2f(d2 * k.a[0]) + k.c;
+  } else {
+    const float base = k.cp / (d2 + k.c);
+    if (k.eint > 0) {
+      float r = 1.f, b = base;
+      for (int e = k.eint; e; e >>= 1) {
+        if (e & 1) r *= b;
+        b *= b;
+      }
+      return r;
+    }
+    return __builtin_amdgcn_exp2f(k.e * __builtin_amdgcn_logf(base));
+  }
+}
+
+template <int KT>
+__device__ __forceinline__ double weight64(const KParams& k, double d2) {
+  if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
+    double w = 0.0;
+    for (int s = 0; s < k.ns; ++s) w += exp(d2 * k.ad[s]) * k.bd[s];
+    return w;
+  } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN) {
+    return exp(d2 * k.ad[0]) + (double)k.c;
+  } else {
+    return pow((double)k.cp, (double)k.e) / pow(d2 + (double)k.c, (double)k.e);
+  }
+}
+
+// Level blending (1 - cos(pi x)) / 2 (CPUUtils.cpp:400-408, Fastapss.cu:84-95).  The FP32
+// form uses the hardware cosine, which takes revolutions: cos(pi x) = cos_rev(x / 2).
+__device__ __forceinline__ float blend(float x) { return 0.5f - 0.5f * __builtin_amdgcn_cosf(0.5f * x); }
+__device__ __forceinline__ double blend(double x) { return (1.0 - cos(3.14159265 * x)) * 0.5; }
+
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+
+// ----------------------------------------------------------------------------- accumulators
+struct KSum {
+  float s, c;
+  __device__ __forceinline__ void init() { s = 0.f, c = 0.f; }
+  __device__ __forceinline__ void add(float x) {
+    const float y = __fsub_rn(x, c);
+    const float t = __fadd_rn(s, y);
+    c = __fsub_rn(__fsub_rn(t, s), y);
+    s = t;
+  }
+  __device__ __forceinline__ double val() const { return (double)s - (double)c; }
+};
+
+template <class T>
+struct V3 {
+  T x, y, z;
+};
+
+// FORM 0: plain global-frame FP32 sums (apssStatsGPU, GPUUtils.cu:318-392)
+struct AccP32 {
+  float w, px, py, pz, nx, ny, nz, pn, pp;
+  __device__ __forceinline__ void init() { w = px = py = pz = nx = ny = nz = pn = pp = 0.f; }
+};
+// FORM 1: compensated, query-frame
+struct AccC32 {
+  KSum w, px, py, pz, nx, ny, nz, pn, pp;
+  __device__ __forceinline__ void init() {
+    w.init(), px.init(), py.init(), pz.init(), nx.init(), ny.init(), nz.init(), pn.init(), pp.init();
+  }
+};
+// FORM 2: plain FP64
+struct AccP64 {
+  double w, px, py, pz, nx, ny, nz, pn, pp;
+  __device__ __forceinline__ void init() { w = px = py = pz = nx = ny = nz = pn = pp = 0.0; }
+};
+
+template <int FORM> struct FormT;
+template <> struct FormT<0> { using T = float;  using Acc = AccP32; };
+template <> struct FormT<1> { using T = float;  using Acc = AccC32; };
+template <> struct FormT<2> { using T = double; using Acc = AccP64; };
+
+struct Counters {
+  uint32_t far_, leafPts, tested, nearN;
+};
+
+// Contribution of one point (leaf member) with multiplier f, weight at the point's mean
+// position (a p) / a (Fastapss.cu:193; CPUUtils.cpp:455).
+template <int FORM, int KT>
+__device__ __forceinline__ void add_point(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv, uint32_t j,
+                                          V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f) {
+  const float4 P = tv.pts[j], N = tv.nrm[j];
+  if constexpr (FORM == 0) {
+    const float a = P.w;
+    const float apx = a * P.x, apy = a * P.y, apz = a * P.z;
+    const float dx = apx / a - q.x, dy = apy / a - q.y, dz = apz / a - q.z;
+    const float w = f * weight32<KT>(kp, dx * dx + dy * dy + dz * dz);
+    acc.w = fmaf(w, a, acc.w);
+    acc.px = fmaf(w, apx, acc.px);
+    acc.py = fmaf(w, apy, acc.py);
+    acc.pz = fmaf(w, apz, acc.pz);
+    acc.nx = fmaf(w, a * N.x, acc.nx);
+    acc.ny = fmaf(w, a * N.y, acc.ny);
+    acc.nz = fmaf(w, a * N.z, acc.nz);
+    acc.pn = fmaf(w, a * (P.x * N.x + P.y * N.y + P.z * N.z), acc.pn);
+    acc.pp = fmaf(w, a * (P.x * P.x + P.y * P.y + P.z * P.z), acc.pp);
+  } else if constexpr (FORM == 1) {
+    const float dx = P.x - q.x, dy = P.y - q.y, dz = P.z - q.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float fa = f * weight32<KT>(kp, d2) * P.w;
+    acc.w.add(fa);
+    acc.px.add(fa * dx);
+    acc.py.add(fa * dy);
+    acc.pz.add(fa * dz);
+    acc.nx.add(fa * N.x);
+    acc.ny.add(fa * N.y);
+    acc.nz.add(fa * N.z);
+    acc.pn.add(fa * (dx * N.x + dy * N.y + dz * N.z));
+    acc.pp.add(fa * d2);
+  } else {
+    const double a = P.w, x = P.x, y = P.y, z = P.z;
+    const double apx = a * x, apy = a * y, apz = a * z;
+    const double dx = apx / a - q.x, dy = apy / a - q.y, dz = apz / a - q.z;
+    const double w = f * weight64<KT>(kp, dx * dx + dy * dy + dz * dz);
+    acc.w += w * a;
+    acc.px += w * apx;
+    acc.py += w * apy;
+    acc.pz += w * apz;
+    acc.nx += w * (a * N.x);
+    acc.ny += w * (a * N.y);
+    acc.nz += w * (a * N.z);
+    acc.pn += w * (a * ((x * N.x + y * N.y) + z * N.z));
+    acc.pp += w * (a * ((x * x + y * y) + z * z));
+  }
+}
+
+// Contribution of an internal node's moments with multiplier f, weight at its mean point.
+template <int FORM, int KT>
+__device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv, uint32_t c,
+                                         float Af, V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f) {
+  if constexpr (FORM == 0) {
+    const float4 m0 = tv.momP[2 * (size_t)c], m1 = tv.momP[2 * (size_t)c + 1];
+    const float dx = m0.x / Af - q.x, dy = m0.y / Af - q.y, dz = m0.z / Af - q.z;
+    const float w = f * weight32<KT>(kp, dx * dx + dy * dy + dz * dz);
+    acc.w = fmaf(w, Af, acc.w);
+    acc.px = fmaf(w, m0.x, acc.px);
+    acc.py = fmaf(w, m0.y, acc.py);
+    acc.pz = fmaf(w, m0.z, acc.pz);
+    acc.nx = fmaf(w, m1.x, acc.nx);
+    acc.ny = fmaf(w, m1.y, acc.ny);
+    acc.nz = fmaf(w, m1.z, acc.nz);
+    acc.pn = fmaf(w, m0.w, acc.pn);
+    acc.pp = fmaf(w, m1.w, acc.pp);
+  } else if constexpr (FORM == 1) {
+    const float4 m0 = tv.momC[2 * (size_t)c], m1 = tv.momC[2 * (size_t)c + 1];
+    const float dx = m0.x - q.x, dy = m0.y - q.y, dz = m0.z - q.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float fw = f * weight32<KT>(kp, d2);
+    const float fa = fw * Af;
+    acc.w.add(fa);
+    acc.px.add(fa * dx);
+    acc.py.add(fa * dy);
+    acc.pz.add(fa * dz);
+    acc.nx.add(fw * m1.x);
+    acc.ny.add(fw * m1.y);
+    acc.nz.add(fw * m1.z);
+    acc.pn.add(fw * (m0.w + (dx * m1.x + dy * m1.y + dz * m1.z)));
+    acc.pp.add(fw * (m1.w + Af * d2));
+  } else {
+    const double4 m0 = tv.momD[2 * (size_t)c], m1 = tv.momD[2 * (size_t)c + 1];
+    const double A = tv.AD[c];
+    const double dx = m0.x / A - q.x, dy = m0.y / A - q.y, dz = m0.z / A - q.z;
+    const double w = f * weight64<KT>(kp, dx * dx + dy * dy + dz * dz);
+    acc.w += w * A;
+    acc.px += w * m0.x;
+    acc.py += w * m0.y;
+    acc.pz += w * m0.z;
+    acc.nx += w * m1.x;
+    acc.ny += w * m1.y;
+    acc.nz += w * m1.z;
+    acc.pn += w * m0.w;
+    acc.pp += w * m1.w;
+  }
+}
+
+template <int FORM, int SEM, int KT, bool CNT>
+__device__ __forceinline__ void add_leaf(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv, uint4 h,
+                                         V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f, Counters& ct) {
+  const uint32_t first = h.x, cnt = hdr_count(h.y);
+  if constexpr (SEM == MLOD_SEM_REF_CPU) {
+    // apssStats of a leaf is assigned, not summed, per point: only the last point stays
+    // (CPUUtils.cpp:448-457).
+    add_point<FORM, KT>(acc, kp, tv, first + cnt - 1, q, f);
+    if constexpr (CNT) ct.leafPts += 1;
+  } else {
+    for (uint32_t j = first; j < first + cnt; ++j) add_point<FORM, KT>(acc, kp, tv, j, q, f);
+    if constexpr (CNT) ct.leafPts += cnt;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T dist3(V3<T> q, float4 g) {
+  const T dx = q.x - (T)g.x, dy = q.y - (T)g.y, dz = q.z - (T)g.z;
+  return fsqrt(dx * dx + dy * dy + dz * dz);
+}
+
+template <class T>
+struct LevelState {
+  uint32_t (*first)[PBS];
+  uint32_t (*mask)[PBS];
+  T (*mul)[PBS];
+  T (*dist)[PBS];
+};
+
+// Expands internal node i (near, or above minDepth) into level L of the lane's state.
+template <int FORM, int SEM, int KT, bool CNT>
+__device__ __forceinline__ void expand(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv, uint32_t i,
+                                       uint4 h, float4 g, typename FormT<FORM>::T dq, typename FormT<FORM>::T m,
+                                       V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff, uint32_t minDepth,
+                                       LevelState<typename FormT<FORM>::T> st, int L, int tid, Counters& ct) {
+  using T = typename FormT<FORM>::T;
+  const uint32_t first = h.x, nc = hdr_count(h.y);
+  const bool blendNode = hdr_depth(h.y) >= minDepth;
+  const T distP = blendNode ? fabs(dq - (T)g.w * cutoff) : T(-1);
+  const float Ai = __uint_as_float(h.z);
+  T shareSum = 0;
+  uint32_t mask = 0;
+  for (uint32_t k = 0; k < nc; ++k) {
+    const uint32_t c = first + k;
+    const float4 gc = tv.geo[c];
+    const uint4 hc = tv.hdr[c];
+    const T dcq = dist3(q, gc);
+    T mc = m;
+    if (blendNode) {
+      const T dc = fmax(dcq - (T)gc.w * cutoff, T(0));
+      const T beta = blend(distP / (distP + dc));
+      mc = m * beta;
+      const T share = SEM == MLOD_SEM_REF_CPU ? T(1) / (T)nc : (T)__uint_as_float(hc.z) / (T)Ai;
+      shareSum += (T(1) - beta) * share;
+    }
+    if constexpr (CNT) ct.tested += 1;
+    if (hdr_leaf(hc.y)) {
+      add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, hc, q, mc, ct);
+    } else if (hdr_depth(hc.y) >= minDepth && dcq > (T)gc.w * cutoff) {
+      add_node<FORM, KT>(acc, kp, tv, c, __uint_as_float(hc.z), q, mc);
+      if constexpr (CNT) ct.far_ += 1;
+    } else {
+      mask |= 1u << k;
+    }
+  }
+  if (blendNode) {
+    if (shareSum != T(0)) add_node<FORM, KT>(acc, kp, tv, i, Ai, q, m * shareSum);
+    if constexpr (CNT) ct.nearN += 1;
+  }
+  st.first[L][tid] = first;
+  st.mask[L][tid] = mask;
+  st.mul[L][tid] = m;
+  st.dist[L][tid] = distP;
+}
+
+template <int FORM, int SEM, int KT, bool CNT>
+__device__ void traverse(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv,
+                         V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff, uint32_t minDepth,
+                         LevelState<typename FormT<FORM>::T> st, int tid, Counters& ct) {
+  using T = typename FormT<FORM>::T;
+  const uint4 h0 = tv.hdr[0];
+  if (hdr_leaf(h0.y)) {
+    add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, h0, q, T(1), ct);
+    return;
+  }
+  const float4 g0 = tv.geo[0];
+  const T d0 = dist3(q, g0);
+  if (minDepth == 0 && d0 > (T)g0.w * cutoff) {
+    add_node<FORM, KT>(acc, kp, tv, 0, __uint_as_float(h0.z), q, T(1));
+    if constexpr (CNT) ct.far_ += 1;
+    return;
+  }
+  expand<FORM, SEM, KT, CNT>(acc, kp, tv, 0, h0, g0, d0, T(1), q, cutoff, minDepth, st, 0, tid, ct);
+  int lvl = 0;
+  while (lvl >= 0) {
+    const uint32_t mask = st.mask[lvl][tid];
+    if (mask == 0) {
+      --lvl;
+      continue;
+    }
+    st.mask[lvl][tid] = mask & (mask - 1u);
+    const uint32_t c = st.first[lvl][tid] + (uint32_t)__builtin_ctz(mask);
+    const float4 gc = tv.geo[c];
+    const uint4 hc = tv.hdr[c];
+    const T dcq = dist3(q, gc);
+    T mc = st.mul[lvl][tid];
+    const T dp = st.dist[lvl][tid];
+    if (dp >= T(0)) {
+      const T dc = fmax(dcq - (T)gc.w * cutoff, T(0));
+      mc *= blend(dp / (dp + dc));
+    }
+    expand<FORM, SEM, KT, CNT>(acc, kp, tv, c, hc, gc, dcq, mc, q, cutoff, minDepth, st, lvl + 1, tid, ct);
+    ++lvl;
+  }
+}
+
+// ----------------------------------------------------------------------------- sphere fit
+// Algebraic sphere fit and projection, Fastapss.cu:354-395 (REF_GPU: plane below |u4| 1e-6,
+// normalize(0) = 0) and CPUUtils.cpp:739-782 (REF_CPU: plane below 1e-12, normalize(0) = NaN).
+template <class T, int SEM>
+__device__ __forceinline__ V3<T> nrmz(V3<T> v) {
+  const T l2 = v.x * v.x + v.y * v.y + v.z * v.z;
+  if (SEM == MLOD_SEM_REF_GPU && l2 == T(0)) return V3<T>{0, 0, 0};
+  const T inv = T(1) / fsqrt(l2);
+  return V3<T>{v.x * inv, v.y * inv, v.z * inv};
+}
+
+template <class T>
+__device__ __forceinline__ T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// Sums in frame `o` (origin of the accumulated positions): q is the query in that frame.
+// Returns the projection in that frame (NaN guard and step clamp applied).
+template <class T, class TF, int SEM>
+__device__ __forceinline__ void fit(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T> q, T stepMax, V3<T>& outP, V3<T>& outN) {
+  const T A = spn - dot3(sp, V3<T>{sn.x / sw, sn.y / sw, sn.z / sw});
+  const T B = spp - dot3(sp, V3<T>{sp.x / sw, sp.y / sw, sp.z / sw});
+  const T u4 = (T)((0.5 * (double)A) / (double)B);
+  const T two_u4 = T(2) * u4;
+  const V3<T> u{(sn.x - two_u4 * sp.x) / sw, (sn.y - two_u4 * sp.y) / sw, (sn.z - two_u4 * sp.z) / sw};
+  const T u0 = -(dot3(sp, u) + u4 * spp) / sw;
+  V3<T> p = q, nr;
+  const double tau = SEM == MLOD_SEM_REF_GPU ? 0.000001 : 0.000000000001;
+  if (fabs((double)u4) < tau) {
+    const V3<T> nn{-u.x, -u.y, -u.z};
+    const T lambda = (u0 - dot3(p, nn)) / dot3(nn, nn);
+    p = V3<T>{p.x + lambda * nn.x, p.y + lambda * nn.y, p.z + lambda * nn.z};
+    nr = nrmz<T, SEM>(sn);
+  } else {
+    const T m2 = T(-2) * u4;
+    const V3<T> c{u.x / m2, u.y / m2, u.z / m2};
+    const T v = dot3(c, c) - u0 / u4;
+    const T r = fsqrt(v > T(0) ? v : T(0));
+    const V3<T> pc = nrmz<T, SEM>(V3<T>{p.x - c.x, p.y - c.y, p.z - c.z});
+    p = V3<T>{c.x + r * pc.x, c.y + r * pc.y, c.z + r * pc.z};
+    nr = nrmz<T, SEM>(V3<T>{u.x + two_u4 * p.x, u.y + two_u4 * p.y, u.z + two_u4 * p.z});
+  }
+  if (isnan(p.x) || isnan(p.y) || isnan(p.z)) {
+    outP = q;
+    outN = V3<T>{0, 0, 0};
+    return;
+  }
+  const V3<T> dlt{p.x - q.x, p.y - q.y, p.z - q.z};
+  const T step = fsqrt(dot3(dlt, dlt));
+  if (step > stepMax) {
+    const T s = stepMax / step;
+    p = V3<T>{q.x + s * dlt.x, q.y + s * dlt.y, q.z + s * dlt.z};
+  }
+  outP = p;
+  outN = nr;
+}
+
+// One projection step of query q (double carried between steps).
+template <int FORM, int SEM>
+__device__ __forceinline__ void fit_step(const typename FormT<FORM>::Acc& acc, V3<double>& qd, V3<float> qf, float stepMax,
+                                         float4& nout) {
+  if constexpr (FORM == 0) {
+    V3<float> p, n;
+    fit<float, float, SEM>(acc.w, V3<float>{acc.px, acc.py, acc.pz}, V3<float>{acc.nx, acc.ny, acc.nz}, acc.pn, acc.pp, qf,
+                           stepMax, p, n);
+    qd = V3<double>{p.x, p.y, p.z};
+    nout = make_float4(n.x, n.y, n.z, 0.f);
+  } else if constexpr (FORM == 1) {
+    // query frame: the query is the origin
+    V3<double> p, n;
+    fit<double, double, SEM>(acc.w.val(), V3<double>{acc.px.val(), acc.py.val(), acc.pz.val()},
+                             V3<double>{acc.nx.val(), acc.ny.val(), acc.nz.val()}, acc.pn.val(), acc.pp.val(),
+                             V3<double>{0, 0, 0}, (double)stepMax, p, n);
+    qd = V3<double>{(double)qf.x + p.x, (double)qf.y + p.y, (double)qf.z + p.z};
+    nout = make_float4((float)n.x, (float)n.y, (float)n.z, 0.f);
+  } else {
+    V3<double> p, n;
+    fit<double, double, SEM>(acc.w, V3<double>{acc.px, acc.py, acc.pz}, V3<double>{acc.nx, acc.ny, acc.nz}, acc.pn, acc.pp,
+                             qd, (double)stepMax, p, n);
+    qd = p;
+    nout = make_float4((float)n.x, (float)n.y, (float)n.z, 0.f);
+  }
+}
+
+// ----------------------------------------------------------------------------- kernels
+// MLoD projection of the sorted queries q[0..m) in place (projectCuda, Fastapss.cu:398-437):
+// n_iter steps per query; lanes whose caller index perm[i] >= limit are skipped.
+template <int FORM, int SEM, int KT, bool CNT>
+__global__ __launch_bounds__(PBS) void k_project(TreeView tv, KParams kp, float4* __restrict__ q, float4* __restrict__ qn,
+                                                 const uint32_t* __restrict__ perm, uint32_t m, uint32_t limit,
+                                                 uint32_t nIter, float stepMax, float cutoff, uint32_t minDepth,
+                                                 unsigned long long* __restrict__ counters) {
+  using T = typename FormT<FORM>::T;
+  __shared__ uint32_t sFirst[kLevels][PBS];
+  __shared__ uint32_t sMask[kLevels][PBS];
+  __shared__ T sMul[kLevels][PBS];
+  __shared__ T sDist[kLevels][PBS];
+  const int tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * PBS + tid;
+  if (i >= m) return;
+  if (limit < m && perm[i] >= limit) return;
+  const LevelState<T> st{sFirst, sMask, sMul, sDist};
+  const float4 q4 = q[i];
+  V3<double> qd{q4.x, q4.y, q4.z};
+  float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
+  Counters ct{0, 0, 0, 0};
+  for (uint32_t it = 0; it < nIter; ++it) {
+    typename FormT<FORM>::Acc acc;
+    acc.init();
+    const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
+    V3<T> qt;
+    if constexpr (FORM == 2)
+      qt = V3<T>{qd.x, qd.y, qd.z};
+    else
+      qt = V3<T>{qf.x, qf.y, qf.z};
+    traverse<FORM, SEM, KT, CNT>(acc, kp, tv, qt, (T)cutoff, minDepth, st, tid, ct);
+    fit_step<FORM, SEM>(acc, qd, qf, stepMax, nout);
+  }
+  q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
+  qn[i] = nout;
+  if constexpr (CNT) {
+    atomicAdd(&counters[0], (unsigned long long)ct.far_);
+    atomicAdd(&counters[1], (unsigned long long)ct.leafPts);
+    atomicAdd(&counters[2], (unsigned long long)ct.tested);
+    atomicAdd(&counters[3], (unsigned long long)ct.nearN);
+  }
+}
+
+// Global APSS over all points (GlobalAPSS.cu:113-192 / APSSStats, CPUUtils.cpp:515-528):
+// point tiles staged through LDS, every lane of the workgroup sums the tile for its query.
+// The fit uses the 1e-12 plane threshold of GlobalAPSS.cu:143 (the REF_CPU fit).
+template <int FORM, int KT>
+__global__ __launch_bounds__(PBS) void k_global(TreeView tv, KParams kp, float4* __restrict__ q, float4* __restrict__ qn,
+                                                const uint32_t* __restrict__ perm, uint32_t m, uint32_t limit,
+                                                uint32_t nIter, float stepMax) {
+  using T = typename FormT<FORM>::T;
+  __shared__ float4 sP[PBS], sN[PBS];
+  const int tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * PBS + tid;
+  const bool active = i < m && !(limit < m && perm[i] >= limit);
+  const float4 q4 = active ? q[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  V3<double> qd{q4.x, q4.y, q4.z};
+  float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t it = 0; it < nIter; ++it) {
+    typename FormT<FORM>::Acc acc;
+    acc.init();
+    const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
+    V3<T> qt;
+    if constexpr (FORM == 2)
+      qt = V3<T>{qd.x, qd.y, qd.z};
+    else
+      qt = V3<T>{qf.x, qf.y, qf.z};
+    for (uint32_t base = 0; base < tv.n; base += PBS) {
+      __syncthreads();
+      if (base + tid < tv.n) {
+        sP[tid] = tv.pts[base + tid];
+        sN[tid] = tv.nrm[base + tid];
+      }
+      __syncthreads();
+      const uint32_t cnt = min((uint32_t)PBS, tv.n - base);
+      if (active) {
+        TreeView lv = tv;
+        lv.pts = sP;
+        lv.nrm = sN;
+        for (uint32_t j = 0; j < cnt; ++j) add_point<FORM, KT>(acc, kp, lv, j, qt, T(1));
+      }
+    }
+    fit_step<FORM, MLOD_SEM_REF_CPU>(acc, qd, qf, stepMax, nout);
+  }
+  if (active) {
+    q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
+    qn[i] = nout;
+  }
+}
+
+template <int FORM, int SEM, int KT, bool CNT>
+void launch_one(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax,
+                unsigned long long* counters, hipStream_t s) {
+  const uint32_t m = t->qs.m;
+  k_project<FORM, SEM, KT, CNT><<<(m + PBS - 1) / PBS, PBS, 0, s>>>(t->view(), kp, t->qs.q, t->qs.qn, t->qs.perm, m, limit,
+                                                                     nIter, stepMax, t->cutoff, t->min_depth, counters);
+}
+
+template <int FORM, int KT>
+void launch_global(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax, hipStream_t s) {
+  const uint32_t m = t->qs.m;
+  k_global<FORM, KT><<<(m + PBS - 1) / PBS, PBS, 0, s>>>(t->view(), kp, t->qs.q, t->qs.qn, t->qs.perm, m, limit, nIter,
+                                                         stepMax);
+}
+
+template <int FORM, int SEM>
+void dispatch_kt(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax, hipStream_t s) {
+  switch (kp.type) {
+    case MLOD_KERNEL_GAUSSIAN: launch_one<FORM, SEM, MLOD_KERNEL_GAUSSIAN, false>(t, kp, limit, nIter, stepMax, nullptr, s); break;
+    case MLOD_KERNEL_SINGULAR: launch_one<FORM, SEM, MLOD_KERNEL_SINGULAR, false>(t, kp, limit, nIter, stepMax, nullptr, s); break;
+    default: launch_one<FORM, SEM, MLOD_KERNEL_GAUSSIAN_MULTIPLE, false>(t, kp, limit, nIter, stepMax, nullptr, s); break;
+  }
+}
+
+template <int FORM>
+void dispatch_global(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax, hipStream_t s) {
+  switch (kp.type) {
+    case MLOD_KERNEL_GAUSSIAN: launch_global<FORM, MLOD_KERNEL_GAUSSIAN>(t, kp, limit, nIter, stepMax, s); break;
+    case MLOD_KERNEL_SINGULAR: launch_global<FORM, MLOD_KERNEL_SINGULAR>(t, kp, limit, nIter, stepMax, s); break;
+    default: launch_global<FORM, MLOD_KERNEL_GAUSSIAN_MULTIPLE>(t, kp, limit, nIter, stepMax, s); break;
+  }
+}
+
+}  // namespace
+
+KParams make_kparams(const mlod_kernel& k) {
+  KParams p{};
+  p.type = k.type;
+  p.ns = k.type == MLOD_KERNEL_GAUSSIAN_MULTIPLE ? k.n_sigmas : 1;
+  p.c = k.constant_shift;
+  p.cp = k.constant_shift > 0.0001f ? k.constant_shift : 0.0001f;
+  p.e = k.exponent;
+  const double er = std::nearbyint((double)k.exponent);
+  p.eint = (er == (double)k.exponent && er >= 1.0 && er <= 16.0) ? (int)er : -1;
+  const double log2e = 1.4426950408889634;
+  if (k.type == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
+    for (int s = 0; s < k.n_sigmas && s < MLOD_MAX_SIGMAS; ++s) {
+      const double sg = k.sigmas[s];
+      p.a[s] = (float)(-log2e / (2.0 * sg * sg));
+      p.b[s] = (float)(-3.0 * std::log2(sg));
+      p.ad[s] = -1.0 / (2.0 * sg * sg);
+      p.bd[s] = 1.0 / (sg * sg * sg);
+    }
+  } else if (k.type == MLOD_KERNEL_GAUSSIAN) {
+    const double sg = k.sigma;
+    p.a[0] = (float)(-log2e / (2.0 * sg * sg));
+    p.ad[0] = -1.0 / (2.0 * sg * sg);
+  }
+  return p;
+}
+
+int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s) {
+  const KParams kp = make_kparams(t->kernel);
+  const uint32_t limit = n;
+  if (t->mode == MLOD_MODE_GLOBAL) {
+    switch (t->precision) {
+      case MLOD_PREC_FP32C: dispatch_global<1>(t, kp, limit, n_iter, step_max, s); break;
+      case MLOD_PREC_FP64: dispatch_global<2>(t, kp, limit, n_iter, step_max, s); break;
+      default: dispatch_global<0>(t, kp, limit, n_iter, step_max, s); break;
+    }
+    LAUNCH_CHECK();
+    return MLOD_OK;
+  }
+  const bool cpu = t->semantics == MLOD_SEM_REF_CPU;
+  switch (t->precision) {
+    case MLOD_PREC_FP32C:
+      if (cpu) dispatch_kt<1, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s);
+      else dispatch_kt<1, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
+      break;
+    case MLOD_PREC_FP64:
+      if (cpu) dispatch_kt<2, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s);
+      else dispatch_kt<2, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
+      break;
+    default:
+      if (cpu) dispatch_kt<0, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s);
+      else dispatch_kt<0, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
+      break;
+  }
+  LAUNCH_CHECK();
+  return MLOD_OK;
+}
+
+// Counter pass: one FP32C step with counting on; positions are not advanced (the step
+// runs on a scratch copy of the queries).
+int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream_t s) {
+  const KParams kp = make_kparams(t->kernel);
+  const bool cpu = t->semantics == MLOD_SEM_REF_CPU;
+#define MLOD_CNT_CASE(SEMV)                                                                                  \
+  switch (kp.type) {                                                                                         \
+    case MLOD_KERNEL_GAUSSIAN: launch_one<1, SEMV, MLOD_KERNEL_GAUSSIAN, true>(t, kp, n, 1, t->radius, d_out4, s); break; \
+    case MLOD_KERNEL_SINGULAR: launch_one<1, SEMV, MLOD_KERNEL_SINGULAR, true>(t, kp, n, 1, t->radius, d_out4, s); break; \
+    default: launch_one<1, SEMV, MLOD_KERNEL_GAUSSIAN_MULTIPLE, true>(t, kp, n, 1, t->radius, d_out4, s); break;     \
+  }
+  if (cpu) {
+    MLOD_CNT_CASE(MLOD_SEM_REF_CPU)
+  } else {
+    MLOD_CNT_CASE(MLOD_SEM_REF_GPU)
+  }
+#undef MLOD_CNT_CASE
+  LAUNCH_CHECK();
+  return MLOD_OK;
+}
+
+}  // namespace mlod

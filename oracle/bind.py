@@ -1,0 +1,272 @@
+"""ctypes bindings for the oracle (liboracle.so) and the reference CPU path (_ref/libmlodref.so).
+
+TEST INFRASTRUCTURE — only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+import this module, and only to check or to time the CPU baseline; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libmlodref.so")
+
+MODES = {"mlod": 0, "global": 1, "knn": 2, "ball": 3}
+SEMANTICS = {"ref_gpu": 0, "ref_cpu": 1}
+
+
+class KernelC(C.Structure):
+    _fields_ = [("type", C.c_int32), ("sigma", C.c_float), ("exponent", C.c_float), ("constant_shift", C.c_float),
+                ("n_sigmas", C.c_int32), ("sigmas", C.c_float * 32)]
+
+
+class ParamsC(C.Structure):
+    _fields_ = [("cutoff", C.c_float), ("min_depth", C.c_uint32), ("semantics", C.c_int32), ("mode", C.c_int32),
+                ("knn", C.c_uint32), ("ball_radius", C.c_float)]
+
+
+def make_kernel(type=2, sigma=0.0, exponent=1.0, constant_shift=0.0, sigmas=()):
+    k = KernelC()
+    k.type = int(type)
+    k.sigma = float(sigma)
+    k.exponent = float(exponent)
+    k.constant_shift = float(constant_shift)
+    k.n_sigmas = len(sigmas)
+    for i, s in enumerate(sigmas):
+        k.sigmas[i] = float(s)
+    return k
+
+
+def make_params(cutoff=2.0, min_depth=0, semantics="ref_cpu", mode="mlod", knn=20, ball_radius=0.1):
+    p = ParamsC()
+    p.cutoff = float(cutoff)
+    p.min_depth = int(min_depth)
+    p.semantics = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+    p.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+    p.knn = int(knn)
+    p.ball_radius = float(ball_radius)
+    return p
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+_vp = C.c_void_p
+_f = C.POINTER(C.c_float)
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int32)
+_u = C.POINTER(C.c_uint32)
+_u64 = C.POINTER(C.c_uint64)
+
+_orc = None
+_ref = None
+
+
+def oracle_lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_tree_build.restype = _vp
+        L.orc_tree_build.argtypes = [_f, _f, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32]
+        L.orc_tree_free.argtypes = [_vp]
+        L.orc_tree_num_nodes.restype = C.c_uint32
+        L.orc_tree_num_nodes.argtypes = [_vp]
+        L.orc_tree_ties.restype = C.c_uint64
+        L.orc_tree_ties.argtypes = [_vp]
+        L.orc_tree_bbox.argtypes = [_vp, _f]
+        L.orc_tree_nodes.argtypes = [_vp, _i, _f, _f, _f, _i, _i, _i, _i, _u, _u64]
+        L.orc_tree_leaf_indices.argtypes = [_vp, _u]
+        L.orc_tree_areas.argtypes = [_vp, _f]
+        L.orc_tree_node_moments.argtypes = [_vp, _f, _d]
+        L.orc_weight.restype = C.c_double
+        L.orc_weight.argtypes = [C.POINTER(KernelC), _f, _f, C.c_int32]
+        L.orc_project.restype = C.c_int32
+        L.orc_project.argtypes = [_vp, C.POINTER(KernelC), C.POINTER(ParamsC), C.c_int32, _f, C.c_uint32, C.c_uint32,
+                                  C.c_float, _d, _d, C.c_int32]
+        L.orc_fit.argtypes = [_d, _d, C.c_double, C.c_int32, _d, _d]
+        _orc = L
+    return _orc
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            raise RuntimeError(f"{REF_SO} missing: run `make -C oracle ref` where the reference sources exist")
+        L = C.CDLL(REF_SO)
+        L.ref_tree_build.restype = _vp
+        L.ref_tree_build.argtypes = [_f, _f, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.ref_tree_free.argtypes = [_vp]
+        L.ref_tree_num_nodes.restype = C.c_uint32
+        L.ref_tree_num_nodes.argtypes = [_vp]
+        L.ref_tree_nodes.argtypes = [_vp, _i, _f, _f, _f, _i, _i, _i, _i, _u, _u64, _u, _f]
+        L.ref_tree_areas.argtypes = [_vp, _f]
+        L.ref_tree_bbox.argtypes = [_vp, _f]
+        L.ref_project.restype = C.c_int32
+        L.ref_project.argtypes = [_vp, C.POINTER(KernelC), C.POINTER(ParamsC), _f, C.c_uint32, C.c_uint32, C.c_float, _f,
+                                  _f, C.c_int32]
+        L.ref_knn.argtypes = [_vp, _f, C.c_uint32, C.c_uint32, _f]
+        _ref = L
+    return _ref
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def _c32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class OracleTree:
+    """The oracle's restated octree (CPUUtils.cpp:540-619)."""
+
+    def __init__(self, points, normals, max_depth=12, max_pts_per_leaf=1, tie_policy=0, seed=12345):
+        self.L = oracle_lib()
+        self.p = _c32(points)
+        self.n = _c32(normals)
+        self.h = self.L.orc_tree_build(_ptr(self.p, C.c_float), _ptr(self.n, C.c_float), len(self.p), max_depth,
+                                       max_pts_per_leaf, tie_policy, seed)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_tree_free(self.h)
+            self.h = None
+
+    @property
+    def num_nodes(self):
+        return int(self.L.orc_tree_num_nodes(self.h))
+
+    @property
+    def ties(self):
+        return int(self.L.orc_tree_ties(self.h))
+
+    def bbox(self):
+        b = np.zeros(6, np.float32)
+        self.L.orc_tree_bbox(self.h, _ptr(b, C.c_float))
+        return b
+
+    def nodes(self):
+        nn = self.num_nodes
+        d = dict(depth=np.zeros(nn, np.int32), bb=np.zeros((nn, 6), np.float32), center=np.zeros((nn, 3), np.float32),
+                 radius=np.zeros(nn, np.float32), father=np.zeros(nn, np.int32), first_child=np.zeros(nn, np.int32),
+                 next_brother=np.zeros(nn, np.int32), num_children=np.zeros(nn, np.int32),
+                 leaf_count=np.zeros(nn, np.uint32), leaf_offset=np.zeros(nn, np.uint64))
+        self.L.orc_tree_nodes(self.h, _ptr(d["depth"], C.c_int32), _ptr(d["bb"], C.c_float), _ptr(d["center"], C.c_float),
+                              _ptr(d["radius"], C.c_float), _ptr(d["father"], C.c_int32),
+                              _ptr(d["first_child"], C.c_int32), _ptr(d["next_brother"], C.c_int32),
+                              _ptr(d["num_children"], C.c_int32), _ptr(d["leaf_count"], C.c_uint32),
+                              _ptr(d["leaf_offset"], C.c_uint64))
+        li = np.zeros(int(d["leaf_count"].sum()), np.uint32)
+        self.L.orc_tree_leaf_indices(self.h, _ptr(li, C.c_uint32))
+        d["leaf_idx"] = li
+        return d
+
+    def areas(self):
+        a = np.zeros(len(self.p), np.float32)
+        self.L.orc_tree_areas(self.h, _ptr(a, C.c_float))
+        return a
+
+    def moments(self):
+        nn = self.num_nodes
+        f = np.zeros((nn, 9), np.float32)
+        d = np.zeros((nn, 9), np.float64)
+        self.L.orc_tree_node_moments(self.h, _ptr(f, C.c_float), _ptr(d, C.c_double))
+        return f, d
+
+    def project(self, queries, kernel, params, n_iter=1, step_max=-1.0, precision=1, nthreads=None):
+        q = _c32(queries)
+        m = len(q)
+        op = np.zeros((m, 3), np.float64)
+        on = np.zeros((m, 3), np.float64)
+        nt = nthreads or os.cpu_count() or 1
+        r = self.L.orc_project(self.h, C.byref(kernel), C.byref(params), int(precision), _ptr(q, C.c_float), m, n_iter,
+                               float(step_max), _ptr(op, C.c_double), _ptr(on, C.c_double), nt)
+        assert r == 0
+        return op, on
+
+
+def oracle_weight(kernel, p, q, precision=1):
+    L = oracle_lib()
+    p = _c32(p)
+    q = _c32(q)
+    return L.orc_weight(C.byref(kernel), _ptr(p, C.c_float), _ptr(q, C.c_float), precision)
+
+
+def oracle_fit(sums9, q, step_max=1e30, ref_gpu=False):
+    L = oracle_lib()
+    s = np.ascontiguousarray(sums9, np.float64)
+    qq = np.ascontiguousarray(q, np.float64)
+    op = np.zeros(3)
+    on = np.zeros(3)
+    L.orc_fit(_ptr(s, C.c_double), _ptr(qq, C.c_double), float(step_max), int(ref_gpu), _ptr(op, C.c_double),
+              _ptr(on, C.c_double))
+    return op, on
+
+
+class RefTree:
+    """The reference's own CPU APSSOctree (compiled into oracle/_ref)."""
+
+    def __init__(self, points, normals, max_depth=12, max_pts_per_leaf=1, seed=12345):
+        self.L = ref_lib()
+        self.p = _c32(points)
+        self.n = _c32(normals)
+        self.h = self.L.ref_tree_build(_ptr(self.p, C.c_float), _ptr(self.n, C.c_float), len(self.p), max_depth,
+                                       max_pts_per_leaf, seed)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_tree_free(self.h)
+            self.h = None
+
+    @property
+    def num_nodes(self):
+        return int(self.L.ref_tree_num_nodes(self.h))
+
+    def bbox(self):
+        b = np.zeros(6, np.float32)
+        self.L.ref_tree_bbox(self.h, _ptr(b, C.c_float))
+        return b
+
+    def nodes(self):
+        nn = self.num_nodes
+        d = dict(depth=np.zeros(nn, np.int32), bb=np.zeros((nn, 6), np.float32), center=np.zeros((nn, 3), np.float32),
+                 radius=np.zeros(nn, np.float32), father=np.zeros(nn, np.int32), first_child=np.zeros(nn, np.int32),
+                 next_brother=np.zeros(nn, np.int32), num_children=np.zeros(nn, np.int32),
+                 leaf_count=np.zeros(nn, np.uint32), leaf_offset=np.zeros(nn, np.uint64),
+                 leaf_idx=np.zeros(len(self.p), np.uint32), moments=np.zeros((nn, 9), np.float32))
+        self.L.ref_tree_nodes(self.h, _ptr(d["depth"], C.c_int32), _ptr(d["bb"], C.c_float), _ptr(d["center"], C.c_float),
+                              _ptr(d["radius"], C.c_float), _ptr(d["father"], C.c_int32),
+                              _ptr(d["first_child"], C.c_int32), _ptr(d["next_brother"], C.c_int32),
+                              _ptr(d["num_children"], C.c_int32), _ptr(d["leaf_count"], C.c_uint32),
+                              _ptr(d["leaf_offset"], C.c_uint64), _ptr(d["leaf_idx"], C.c_uint32),
+                              _ptr(d["moments"], C.c_float))
+        return d
+
+    def areas(self):
+        a = np.zeros(len(self.p), np.float32)
+        self.L.ref_tree_areas(self.h, _ptr(a, C.c_float))
+        return a
+
+    def project(self, queries, kernel, params, n_iter=1, step_max=-1.0, nthreads=None):
+        q = _c32(queries)
+        m = len(q)
+        op = np.zeros((m, 3), np.float32)
+        on = np.zeros((m, 3), np.float32)
+        nt = nthreads or os.cpu_count() or 1
+        r = self.L.ref_project(self.h, C.byref(kernel), C.byref(params), _ptr(q, C.c_float), m, n_iter, float(step_max),
+                               _ptr(op, C.c_float), _ptr(on, C.c_float), nt)
+        assert r == 0
+        return op, on
+
+    def knn(self, queries, k):
+        q = _c32(queries)
+        out = np.zeros((len(q), k, 3), np.float32)
+        self.L.ref_knn(self.h, _ptr(q, C.c_float), len(q), k, _ptr(out, C.c_float))
+        return out

@@ -1,0 +1,188 @@
+"""GPU MLoD / global APSS projection parity.
+
+Tolerances (x bbox diagonal): the reference CPU path (golden vectors) 1e-5 for every form;
+the FP64 oracle with the same semantics 1e-6 for FP32C, 2e-6 for plain FP32, 1e-9 for FP64."""
+import numpy as np
+import pytest
+
+from conftest import MODE_NAMES, golden_kernel, golden_names, load_golden
+from mlodsurfaces_amd import datasets as ds
+
+pytestmark = pytest.mark.gpu
+
+TOL_REF = 1e-5
+TOL_ORACLE = {"fp32": 2e-6, "fp32c": 1e-6, "fp64": 1e-9}
+
+_trees = {}
+
+
+def tree_for(name, p, n):
+    from gpu_util import gpu_tree
+    if name not in _trees:
+        _trees[name] = gpu_tree(p, n)
+    return _trees[name]
+
+
+MLOD_GOLDEN = [n for n in golden_names("proj_") if "knn" not in n and "ball" not in n]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32c", "fp64"])
+@pytest.mark.parametrize("name", MLOD_GOLDEN)
+def test_matches_reference_cpu_golden(oracle_built, name, precision):
+    from gpu_util import gpu_project, rel_err
+    g = load_golden(name)
+    tn = str(g["tree"])
+    t = load_golden(f"tree_{tn}.npz")
+    print("case", name, precision, flush=True)
+    tree = tree_for(tn, t["points"], t["normals"])
+    mode = MODE_NAMES[int(g["mode"])]
+    kern = golden_kernel(g)
+    op, on, _ = gpu_project(tree, g["queries"], kern, cutoff=float(g["cutoff"]), min_depth=int(g["min_depth"]),
+                            semantics="ref_cpu", precision=precision, n_iter=int(g["n_iter"]), mode=mode)
+    diag = 2 * float(t["radius"][0])
+    e = rel_err(op, g["out_points"], diag)
+    assert e.max() < TOL_REF, e.max()
+    assert np.all(np.abs(np.linalg.norm(on, axis=1) - 1) < 1e-4)
+    # and the FP64 oracle with the same semantics
+    o = oracle_built.OracleTree(t["points"], t["normals"])
+    prm = oracle_built.make_params(cutoff=float(g["cutoff"]), min_depth=int(g["min_depth"]), semantics="ref_cpu",
+                                   mode=mode)
+    o64, _ = o.project(g["queries"], oracle_built.make_kernel(**kern), prm, n_iter=int(g["n_iter"]), precision=1)
+    e = rel_err(op, o64, diag)
+    assert e.max() < TOL_ORACLE[precision], e.max()
+
+
+KERNELS = {
+    "mg4": lambda R: dict(type=2, sigmas=[0.01 * 2.0 ** i * R for i in range(4)]),
+    "mg6": lambda R: dict(type=2, sigmas=[0.01 * 2.0 ** i * R for i in range(6)]),
+    "rational": lambda R: dict(type=1, exponent=4.0, constant_shift=1e-3),
+    "rational_e2.5": lambda R: dict(type=1, exponent=2.5, constant_shift=2e-3),
+    "gauss": lambda R: dict(type=0, sigma=0.03 * R, constant_shift=0.0),
+}
+
+
+@pytest.mark.parametrize("semantics", ["ref_gpu", "ref_cpu"])
+@pytest.mark.parametrize("kname", list(KERNELS))
+@pytest.mark.parametrize("precision", ["fp32", "fp32c", "fp64"])
+def test_matches_fp64_oracle(oracle_built, semantics, kname, precision):
+    from gpu_util import gpu_project, rel_err
+    print("case", semantics, kname, precision, flush=True)
+    p, n = ds.torus(40000, seed=31)
+    tree = tree_for("torus40k", p, n)
+    R = ds.bbox_radius(p)
+    kern = KERNELS[kname](R)
+    q = ds.jitter_queries(p[:3000], 0.01, seed=5)
+    op, on, _ = gpu_project(tree, q, kern, cutoff=2.5, semantics=semantics, precision=precision, n_iter=2)
+    o = oracle_built.OracleTree(p, n)
+    o64, n64 = o.project(q, oracle_built.make_kernel(**kern), oracle_built.make_params(cutoff=2.5, semantics=semantics),
+                         n_iter=2, precision=1)
+    e = rel_err(op, o64, 2 * R)
+    assert e.max() < TOL_ORACLE[precision], (e.max(), np.sqrt(np.mean(e ** 2)))
+    assert np.all(np.sum(on * n64, 1) > 0.9999)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32c", "fp64"])
+def test_global_apss_matches_oracle(oracle_built, precision):
+    from gpu_util import gpu_project, rel_err
+    p, n = ds.sphere(6000, seed=32)
+    tree = tree_for("sphere6k", p, n)
+    R = ds.bbox_radius(p)
+    kern = KERNELS["mg4"](R)
+    q = ds.jitter_queries(p[:700], 0.02, seed=6)
+    op, _, _ = gpu_project(tree, q, kern, precision=precision, n_iter=2, mode="global")
+    o = oracle_built.OracleTree(p, n)
+    o64, _ = o.project(q, oracle_built.make_kernel(**kern), oracle_built.make_params(mode="global"), n_iter=2,
+                       precision=1)
+    assert rel_err(op, o64, 2 * R).max() < TOL_ORACLE[precision]
+
+
+def test_in_place_iterations_and_subset():
+    """project(n_iter=3) once equals three project(n_iter=1) calls (the queries stay on the
+    device between calls, Fastapss.cu:450); project(n < m) touches only the first n."""
+    from mlodsurfaces_amd import apss as A
+    p, n = ds.sphere(20000, seed=33)
+    tree = tree_for("sphere20k", p, n)
+    R = ds.bbox_radius(p)
+    K = A.Kernel(**KERNELS["mg4"](R))
+    q = ds.jitter_queries(p[:5000], 0.01, seed=7)
+    ap = A.Fastapss(tree, K)
+    ap.setScalingProtectionSphere(3.0)
+    ap.copyPointsToGPU(len(q), q)
+    a3, _ = ap.project(len(q), np.zeros_like(q), None, 3, K)
+    ap.copyPointsToGPU(len(q), q)
+    for _ in range(3):
+        b3, _ = ap.project(len(q), np.zeros_like(q), None, 1, K)
+    assert np.array_equal(a3, b3)
+    ap.copyPointsToGPU(len(q), q)
+    part, _ = ap.project(1000, np.zeros((1000, 3), np.float32), None, 1, K)
+    full, _ = ap.download()
+    assert np.array_equal(full[1000:], q[1000:])
+    assert np.array_equal(full[:1000], part)
+    ap.copyPointsToGPU(len(q), q)
+    one, _ = ap.project(len(q), np.zeros_like(q), None, 1, K)
+    assert np.array_equal(one[:1000], part)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32c", "fp64"])
+def test_query_order_invariance(precision):
+    """Each query's result is independent of the batch it is in and of its position."""
+    from gpu_util import gpu_project
+    p, n = ds.sphere(20000, seed=33)
+    tree = tree_for("sphere20k", p, n)
+    kern = KERNELS["mg4"](ds.bbox_radius(p))
+    q = ds.jitter_queries(p[:4000], 0.01, seed=8)
+    perm = np.random.default_rng(0).permutation(len(q))
+    a, an, _ = gpu_project(tree, q, kern, cutoff=3.0, precision=precision)
+    b, bn, _ = gpu_project(tree, q[perm], kern, cutoff=3.0, precision=precision)
+    c, cn, _ = gpu_project(tree, q[:777], kern, cutoff=3.0, precision=precision)
+    assert np.array_equal(a[perm], b) and np.array_equal(an[perm], bn)
+    assert np.array_equal(a[:777], c)
+
+
+def test_step_clamp_and_plane_closed_form():
+    """Planar samples with constant normals: u4 = 0, so the plane branch projects orthogonally
+    (CPUUtils.cpp:751-757); a step bound caps the displacement (CPUUtils.cpp:778-781)."""
+    from gpu_util import gpu_project, gpu_tree
+    rng = np.random.default_rng(3)
+    p = np.concatenate([rng.uniform(-1, 1, (8000, 2)), np.zeros((8000, 1))], 1).astype(np.float32)
+    n = np.tile(np.array([0, 0, 1], np.float32), (8000, 1))
+    tree = gpu_tree(p, n)
+    kern = dict(type=2, sigmas=[0.05, 0.1])
+    q = np.concatenate([rng.uniform(-0.5, 0.5, (500, 2)), rng.uniform(-0.05, 0.05, (500, 1))], 1).astype(np.float32)
+    for sem in ("ref_gpu", "ref_cpu"):
+        for prec in ("fp32", "fp32c", "fp64"):
+            op, on, _ = gpu_project(tree, q, kern, cutoff=2.0, semantics=sem, precision=prec)
+            assert np.abs(op[:, 2]).max() < 1e-6
+            assert np.allclose(op[:, :2], q[:, :2], atol=1e-6)
+            assert np.allclose(on, [0, 0, 1], atol=1e-6)
+            op, _, _ = gpu_project(tree, q, kern, cutoff=2.0, semantics=sem, precision=prec, step_max=0.01)
+            step = np.linalg.norm(op - q, axis=1)
+            assert step.max() <= 0.01 * (1 + 1e-5)
+
+
+def test_sphere_closed_form():
+    from gpu_util import gpu_project, gpu_tree
+    p, n = ds.sphere(50000, seed=34, noise=0.0)
+    tree = gpu_tree(p, n)
+    R = ds.bbox_radius(p)
+    u = np.random.default_rng(1).standard_normal((3000, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q = (u * np.random.default_rng(2).uniform(0.97, 1.03, (3000, 1))).astype(np.float32)
+    op, on, _ = gpu_project(tree, q, KERNELS["mg4"](R), cutoff=3.0, precision="fp32c")
+    assert np.abs(np.linalg.norm(op.astype(np.float64), axis=1) - 1).max() < 2e-6
+    assert np.abs(np.sum(on * op, 1) / np.linalg.norm(op, axis=1) - 1).max() < 1e-5
+
+
+def test_state_errors():
+    from mlodsurfaces_amd import apss as A
+    from mlodsurfaces_amd.apss import MlodError
+    p, n = ds.sphere(500)
+    t = A.APSSOctree().build(p, n)
+    ap = A.Fastapss(t)
+    ap.copyPointsToGPU(10, p[:10])
+    with pytest.raises(MlodError):
+        ap.project(10, None, None, 1)  # no kernel installed yet
+    ap.updateKernel(A.Kernel(type=0, sigma=0.1))
+    ap.project(0, None, None, 1)  # empty batch is a no-op (Fastapss.cu:441-445)
+    with pytest.raises(MlodError):
+        ap.project(11, None, None, 1)  # more than uploaded
