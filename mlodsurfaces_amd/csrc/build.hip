@@ -275,7 +275,9 @@ __global__ __launch_bounds__(BS) void k_emit(const uint64_t* __restrict__ codes,
       for (int a = 0; a < 3; ++a) c[a] = __fdiv_rn(__fadd_rn(lo[a], hi[a]), 2.0f);  // BBOX::center, CPUUtils.h:86
       const float dx = __fsub_rn(lo[0], hi[0]), dy = __fsub_rn(lo[1], hi[1]), dz = __fsub_rn(lo[2], hi[2]);
       const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
-      const float r = __fdiv_rn(__fsqrt_rn(d2), 2.0f);  // BBOX::radius, CPUUtils.h:84-85
+      // BBOX::radius, CPUUtils.h:84-85.  sqrt in double then one rounding to float is the
+      // correctly rounded float sqrt (53 >= 2 * 24 + 2 bits); the GPU's f32 sqrt is not.
+      const float r = (float)sqrt((double)d2) * 0.5f;
       geo[node] = make_float4(c[0], c[1], c[2], r);
       hdr[node] = make_uint4(first, ((uint32_t)d << 27) | (leaf ? kLeafBit : 0u), 0u, father);
 #pragma unroll
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(BS) void k_points(const float* __restrict__ xyz, co
   const uint32_t leaf = pointLeaf[i];
   const float diag = __fmul_rn(geo[leaf].w, 2.0f);
   const float leafArea = __fmul_rn(diag, diag);
-  const float a = __fdiv_rn(leafArea, (float)hdr_count(hdr[leaf].y));
+  const float a = (float)((double)leafArea / (double)hdr_count(hdr[leaf].y));  // correctly rounded f32 division
   pts[i] = make_float4(xyz[3 * (uint64_t)src], xyz[3 * (uint64_t)src + 1], xyz[3 * (uint64_t)src + 2], a);
   nrm[i] = make_float4(nrmIn[3 * (uint64_t)src], nrmIn[3 * (uint64_t)src + 1], nrmIn[3 * (uint64_t)src + 2], 0.f);
   areaOrig[src] = a;

@@ -46,7 +46,7 @@ __device__ __forceinline__ float weight32(const KParams& k, float d2) {
   if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
     float w = 0.f;
     for (int s = 0; s < k.ns; ++s) w += __builtin_amdgcn_exp2f(fmaf(d2, k.a[s], k.b[s]));
-    return w;
+    return w == 0.f ? k.c : w;  // an all-underflowed sum falls back to constant_shift (Kernel.cpp:69-73)
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN) {
     return __builtin_amdgcn_exp2f(d2 * k.a[0]) + k.c;
   } else {
@@ -68,7 +68,7 @@ __device__ __forceinline__ double weight64(const KParams& k, double d2) {
   if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
     double w = 0.0;
     for (int s = 0; s < k.ns; ++s) w += exp(d2 * k.ad[s]) * k.bd[s];
-    return w;
+    return w == 0.0 ? (double)k.c : w;
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN) {
     return exp(d2 * k.ad[0]) + (double)k.c;
   } else {
