@@ -1,0 +1,251 @@
+// mlod_apss.hpp — header-only C++ drop-in for the reference's projection plugin interface.
+//
+// Same class names, method names, argument meaning and defaults as the reference's
+// `APSS` (sources/apss/GPUUtils.h:22-67), `Fastapss` (Fastapss.h:32-51), `GlobalAPSS`
+// (GlobalAPSS.h:19-43), `Kernel` (Kernel.h:24-60) and the parts of `APSSOctree`
+// (CPUUtils.h:166-229) the projection path uses, implemented on libmlod.so (mlod.h).
+// Vector arguments take any 3-float type laid out as x, y, z (glm::vec3 included).
+//
+// Differences: errors throw mlod::Error instead of exit() (gpuErrchk, GPUUtils.cu:32-41);
+// the octree is built on the GPU by APSSOctree::build; projectCPU is not provided (the
+// product has no CPU path); set_semantics / set_precision choose the reference function
+// and the arithmetic form (see mlod.h).
+#ifndef MLOD_APSS_HPP
+#define MLOD_APSS_HPP
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mlod.h"
+
+namespace mlod {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int code) {
+  if (code != MLOD_OK) throw Error(code, std::string("libmlod: ") + mlod_last_error());
+}
+
+template <class V>
+inline const float* fptr(const V* v) {
+  static_assert(sizeof(V) == 3 * sizeof(float), "vector type must be three packed floats");
+  return reinterpret_cast<const float*>(v);
+}
+template <class V>
+inline float* fptr(V* v) {
+  static_assert(sizeof(V) == 3 * sizeof(float), "vector type must be three packed floats");
+  return reinterpret_cast<float*>(v);
+}
+
+// Kernel (Kernel.h:24-60); defaults of Kernel::Kernel (Kernel.cpp:13-16).
+class Kernel {
+ public:
+  enum KernelType { GAUSSIAN, SINGULAR, GAUSSIAN_MULTIPLE };
+  float sigma = 2.0f;
+  float exponent = 1.0f;
+  float constant_shift = 0.0f;
+  std::vector<float> sigmas;
+  std::vector<float> fixedSigmas;
+  float beginningSigma = 0.0f;
+  float factor = 0.0f;
+  unsigned nbOfComputedSigmas = 1;
+  KernelType type = GAUSSIAN;
+
+  // Kernel::updateSigmas (Kernel.cpp:110-128).
+  void updateSigmas(bool computedSigmas = true, float radius = 1.f) {
+    if (computedSigmas) {
+      sigmas.resize(nbOfComputedSigmas);
+      sigmas[0] = beginningSigma;
+      for (unsigned i = 1; i < nbOfComputedSigmas; i++) sigmas[i] = sigmas[i - 1] * factor;
+    } else {
+      sigmas = fixedSigmas;
+    }
+    sigma *= radius;
+    for (float& s : sigmas) s *= radius;
+  }
+
+  mlod_kernel to_c() const {
+    mlod_kernel k{};
+    k.type = (int32_t)type;
+    k.sigma = sigma;
+    k.exponent = exponent;
+    k.constant_shift = constant_shift;
+    if (sigmas.size() > MLOD_MAX_SIGMAS) throw Error(MLOD_ERR_INVALID, "at most 32 sigmas");
+    k.n_sigmas = (int32_t)sigmas.size();
+    for (size_t i = 0; i < sigmas.size(); ++i) k.sigmas[i] = sigmas[i];
+    return k;
+  }
+};
+
+// The octree (APSSOctree, CPUUtils.h:166-229), built and resident on the GPU.
+class APSSOctree {
+ public:
+  APSSOctree() = default;
+  APSSOctree(const APSSOctree&) = delete;
+  APSSOctree& operator=(const APSSOctree&) = delete;
+  ~APSSOctree() { mlod_tree_destroy(m_tree); }
+
+  // APSSOctree::build (CPUUtils.h:176-180); exact-midpoint ties go to tie_policy's child.
+  template <class V>
+  void build(const std::vector<V>& i_points, const std::vector<V>& normals, bool updateRootBoundingBox = true,
+             unsigned maxDepth = 12, unsigned maxNumberOfPointsPerLeaf = 1, int tie_policy = 0) {
+    if (!updateRootBoundingBox) throw Error(MLOD_ERR_UNSUPPORTED, "a fixed root bounding box is not supported");
+    if (i_points.size() != normals.size()) throw Error(MLOD_ERR_INVALID, "points and normals differ in length");
+    mlod_tree_destroy(m_tree);
+    m_tree = nullptr;
+    const mlod_build_opts o{maxDepth, maxNumberOfPointsPerLeaf, tie_policy};
+    check(mlod_tree_create(fptr(i_points.data()), fptr(normals.data()), (uint32_t)i_points.size(), &o, nullptr, &m_tree));
+  }
+
+  mlod_tree_info info() const {
+    mlod_tree_info i{};
+    check(mlod_tree_get_info(m_tree, &i));
+    return i;
+  }
+  float radius() const { return info().radius; }
+  unsigned getNbOfLeaves() const { return info().n_points; }
+
+  float getScalingProtectionSphere() const { return m_scalingProtectionSphere; }
+  void setScalingProtectionSphere(float v) { m_scalingProtectionSphere = v; }
+  unsigned getMinDepth() const { return m_minDepth; }
+  void setMinDepth(unsigned v) { m_minDepth = v; }
+
+  // APSSOctree::knnMode / ballMode (CPUUtils.cpp:674-688).
+  void knnMode(bool setMode, unsigned knn = 20) {
+    m_knnMode = setMode;
+    if (m_knnMode) m_ballMode = false;
+    m_knn = knn;
+  }
+  void ballMode(bool setMode, float ballRadius = 0.1f) {
+    m_ballMode = setMode;
+    if (m_ballMode) m_knnMode = false;
+    m_ballRadius = ballRadius;
+  }
+  bool isAPSS() const { return m_knnMode || m_ballMode; }
+
+  mlod_tree* handle() const {
+    if (!m_tree) throw Error(MLOD_ERR_STATE, "octree not built");
+    return m_tree;
+  }
+
+ private:
+  friend class APSS;
+  mlod_tree* m_tree = nullptr;
+  float m_scalingProtectionSphere = 2.0f;
+  unsigned m_minDepth = 0;
+  bool m_knnMode = false;
+  unsigned m_knn = 20;
+  bool m_ballMode = false;
+  float m_ballRadius = 0.1f;
+};
+
+// The projection plugin interface (GPUUtils.h:22-67).
+class APSS {
+ public:
+  virtual ~APSS() { erasePointsFromGPU(); }
+
+  // APSS::project (GPUUtils.h:26-27): n_iterations steps in place on the uploaded queries;
+  // `kernel` is ignored as in Fastapss::project (the installed kernel is used).
+  virtual void project(unsigned nbOfVectors, float* outputPoints, float* outputNormals, unsigned n_iterations,
+                       const Kernel& kernel, float stepMaxSize = -1.f) const {
+    (void)kernel;
+    sync();
+    check(mlod_project(m_apssOctree->handle(), nbOfVectors, n_iterations, stepMaxSize, outputPoints, outputNormals, m_stream));
+  }
+  template <class V>
+  void project(unsigned nbOfVectors, V* outputPoints, V* outputNormals, unsigned n_iterations, const Kernel& kernel,
+               float stepMaxSize = -1.f) const {
+    project(nbOfVectors, outputPoints ? fptr(outputPoints) : nullptr, outputNormals ? fptr(outputNormals) : nullptr,
+            n_iterations, kernel, stepMaxSize);
+  }
+
+  // Fastapss::copyToGPU (Fastapss.cu:457-484): the tree is already resident; installs the kernel.
+  virtual void copyToGPU(const Kernel& kernel, unsigned nbOfVectors) {
+    (void)nbOfVectors;
+    updateKernel(kernel);
+  }
+  virtual void eraseFromGPU() { erasePointsFromGPU(); }
+
+  // APSS::copyPointsToGPU / erasePointsFromGPU (GPUUtils.cu:406-411, 452-464).
+  virtual void copyPointsToGPU(unsigned nbOfVectors, const float* points) {
+    check(mlod_queries_upload(m_apssOctree->handle(), points, nbOfVectors, m_stream));
+  }
+  template <class V>
+  void copyPointsToGPU(unsigned nbOfVectors, const V* points) {
+    copyPointsToGPU(nbOfVectors, fptr(points));
+  }
+  virtual void erasePointsFromGPU() {
+    if (m_apssOctree && m_apssOctree->m_tree) mlod_queries_erase(m_apssOctree->m_tree);
+  }
+
+  // APSS::updateKernel (GPUUtils.cu:466-472).
+  void updateKernel(const Kernel& kernel) {
+    const mlod_kernel k = kernel.to_c();
+    check(mlod_set_kernel(m_apssOctree->handle(), &k));
+  }
+
+  void setScalingProtectionSphere(float val) {
+    m_scalingProtectionSphere = val;
+    m_apssOctree->setScalingProtectionSphere(val);
+  }
+  float getScalingProtectionSphere() const { return m_scalingProtectionSphere; }
+  void setMinDepth(unsigned val) {
+    m_minDepth = val;
+    m_apssOctree->setMinDepth(val);
+  }
+  unsigned getMinDepth() const { return m_minDepth; }
+
+  void set_semantics(int semantics) { m_semantics = semantics; }
+  void set_precision(int precision) { m_precision = precision; }
+  void set_stream(void* hip_stream) { m_stream = hip_stream; }
+  const APSSOctree* CPU() const { return m_apssOctree; }
+
+ protected:
+  APSS(APSSOctree* octree, int mode) : m_apssOctree(octree), m_mode(mode) {
+    m_scalingProtectionSphere = octree->getScalingProtectionSphere();
+    m_minDepth = octree->getMinDepth();
+  }
+  void sync() const {
+    mlod_tree* t = m_apssOctree->handle();
+    check(mlod_set_params(t, m_scalingProtectionSphere, m_minDepth, m_semantics, m_precision));
+    int mode = m_mode;
+    if (mode == MLOD_MODE_MLOD && m_apssOctree->m_knnMode) mode = MLOD_MODE_KNN;
+    if (mode == MLOD_MODE_MLOD && m_apssOctree->m_ballMode) mode = MLOD_MODE_BALL;
+    check(mlod_set_mode(t, mode, m_apssOctree->m_knn, m_apssOctree->m_ballRadius));
+  }
+
+  APSSOctree* m_apssOctree;
+  int m_mode;
+  float m_scalingProtectionSphere = 2.f;
+  unsigned m_minDepth = 0;
+  int m_semantics = MLOD_SEM_REF_GPU;
+  int m_precision = MLOD_PREC_FP32;
+  void* m_stream = nullptr;
+};
+
+// MLoD projection (Fastapss, Fastapss.h:32-51).
+class Fastapss : public APSS {
+ public:
+  explicit Fastapss(APSSOctree* apssOctree) : APSS(apssOctree, MLOD_MODE_MLOD) {}
+  Fastapss(APSSOctree* apssOctree, const Kernel& kernel, unsigned nbOfVectors) : Fastapss(apssOctree) {
+    copyToGPU(kernel, nbOfVectors);
+  }
+};
+
+// Brute-force APSS over all points (GlobalAPSS, GlobalAPSS.h:19-43).
+class GlobalAPSS : public APSS {
+ public:
+  explicit GlobalAPSS(APSSOctree* apssOctree) : APSS(apssOctree, MLOD_MODE_GLOBAL) {}
+  GlobalAPSS(APSSOctree* apssOctree, const Kernel& kernel, unsigned nbOfVectors) : GlobalAPSS(apssOctree) {
+    copyToGPU(kernel, nbOfVectors);
+  }
+};
+
+}  // namespace mlod
+
+#endif  // MLOD_APSS_HPP

@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmlod.so")
+# MLOD_LIB_PATH selects another build of the same library (e.g. an alternative kernel variant).
+LIB_PATH = os.environ.get("MLOD_LIB_PATH") or os.path.join(HERE, "libmlod.so")
 
 MLOD_OK = 0
 MAX_SIGMAS = 32

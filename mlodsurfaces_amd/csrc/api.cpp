@@ -187,11 +187,15 @@ int mlod_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, floa
   if (n == 0) return MLOD_OK;  // Fastapss::project returns on an empty batch (Fastapss.cu:441-445)
   if (!t->kernel_set) return fail(MLOD_ERR_STATE, "no kernel set (mlod_set_kernel)");
   if (n > t->qs.m) return fail(MLOD_ERR_INVALID, "n exceeds the number of uploaded queries");
-  if (t->mode == MLOD_MODE_KNN || t->mode == MLOD_MODE_BALL)
-    return fail(MLOD_ERR_UNSUPPORTED, "kNN / ball APSS is not built into this library yet");
   if (step_max < 0.f) step_max = t->radius;  // Fastapss.cu:448-449
   hipStream_t s = (hipStream_t)stream;
-  int r = n_iter ? launch_project(t, n, n_iter, step_max, s) : MLOD_OK;
+  int r = MLOD_OK;
+  if (n_iter) {
+    // kNN / ball APSS take no step bound: their fit is called with q aliased to the output
+    // (CPUUtils.cpp:735), which makes the clamp a no-op.
+    r = (t->mode == MLOD_MODE_KNN || t->mode == MLOD_MODE_BALL) ? launch_neighbors(t, n, n_iter, s)
+                                                                 : launch_project(t, n, n_iter, step_max, s);
+  }
   if (r) return r;
   if (out_pts || out_nmls) return queries_get(t, n, out_pts, out_nmls, s);
   return MLOD_OK;

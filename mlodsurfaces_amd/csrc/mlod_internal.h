@@ -134,6 +134,9 @@ KParams make_kparams(const mlod_kernel& k);
 int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s);
 int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream_t s);
 
+// neighbors.hip
+int launch_neighbors(mlod_tree* t, uint32_t n, uint32_t n_iter, hipStream_t s);
+
 // queries.hip
 int queries_set(mlod_tree* t, const float* xyz, uint32_t m, bool device_src, hipStream_t s);
 int queries_get(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, hipStream_t s);

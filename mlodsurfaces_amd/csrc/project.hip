@@ -45,7 +45,14 @@ template <int KT>
 __device__ __forceinline__ float weight32(const KParams& k, float d2) {
   if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
     float w = 0.f;
-    for (int s = 0; s < k.ns; ++s) w += __builtin_amdgcn_exp2f(fmaf(d2, k.a[s], k.b[s]));
+    if (k.ns <= 8) {
+      // statically indexed, so the coefficients are loaded once into scalar registers
+#pragma unroll
+      for (int s = 0; s < 8; ++s)
+        if (s < k.ns) w += __builtin_amdgcn_exp2f(fmaf(d2, k.a[s], k.b[s]));
+    } else {
+      for (int s = 0; s < k.ns; ++s) w += __builtin_amdgcn_exp2f(fmaf(d2, k.a[s], k.b[s]));
+    }
     return w == 0.f ? k.c : w;  // an all-underflowed sum falls back to constant_shift (Kernel.cpp:69-73)
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN) {
     return __builtin_amdgcn_exp2f(d2 * k.a[0]) + k.c;
@@ -252,57 +259,19 @@ __device__ __forceinline__ T dist3(V3<T> q, float4 g) {
 template <class T>
 struct LevelState {
   uint32_t (*first)[PBS];
-  uint32_t (*mask)[PBS];
+  uint8_t (*mask)[PBS];
   T (*mul)[PBS];
   T (*dist)[PBS];
 };
 
-// Expands internal node i (near, or above minDepth) into level L of the lane's state.
-template <int FORM, int SEM, int KT, bool CNT>
-__device__ __forceinline__ void expand(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv, uint32_t i,
-                                       uint4 h, float4 g, typename FormT<FORM>::T dq, typename FormT<FORM>::T m,
-                                       V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff, uint32_t minDepth,
-                                       LevelState<typename FormT<FORM>::T> st, int L, int tid, Counters& ct) {
-  using T = typename FormT<FORM>::T;
-  const uint32_t first = h.x, nc = hdr_count(h.y);
-  const bool blendNode = hdr_depth(h.y) >= minDepth;
-  const T distP = blendNode ? fabs(dq - (T)g.w * cutoff) : T(-1);
-  const float Ai = __uint_as_float(h.z);
-  T shareSum = 0;
-  uint32_t mask = 0;
-  for (uint32_t k = 0; k < nc; ++k) {
-    const uint32_t c = first + k;
-    const float4 gc = tv.geo[c];
-    const uint4 hc = tv.hdr[c];
-    const T dcq = dist3(q, gc);
-    T mc = m;
-    if (blendNode) {
-      const T dc = fmax(dcq - (T)gc.w * cutoff, T(0));
-      const T beta = blend(distP / (distP + dc));
-      mc = m * beta;
-      const T share = SEM == MLOD_SEM_REF_CPU ? T(1) / (T)nc : (T)__uint_as_float(hc.z) / (T)Ai;
-      shareSum += (T(1) - beta) * share;
-    }
-    if constexpr (CNT) ct.tested += 1;
-    if (hdr_leaf(hc.y)) {
-      add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, hc, q, mc, ct);
-    } else if (hdr_depth(hc.y) >= minDepth && dcq > (T)gc.w * cutoff) {
-      add_node<FORM, KT>(acc, kp, tv, c, __uint_as_float(hc.z), q, mc);
-      if constexpr (CNT) ct.far_ += 1;
-    } else {
-      mask |= 1u << k;
-    }
-  }
-  if (blendNode) {
-    if (shareSum != T(0)) add_node<FORM, KT>(acc, kp, tv, i, Ai, q, m * shareSum);
-    if constexpr (CNT) ct.nearN += 1;
-  }
-  st.first[L][tid] = first;
-  st.mask[L][tid] = mask;
-  st.mul[L][tid] = m;
-  st.dist[L][tid] = distP;
-}
-
+// Flattened traversal.  The lane's current node (expanded: near, or above minDepth) has its
+// children tested one per loop iteration; when they are done the node's own (1 - b_c) share
+// is added and the next pending near child anywhere on the lane's path becomes the current
+// node.  Either way an iteration handles exactly one node X: it loads X, computes its distance
+// and its blend factor against its parent, then either classifies X as a child (far / leaf /
+// near) or makes it the new current node, and ends in one shared add_node.  Lanes of a
+// wavefront therefore advance by one node per iteration whatever their position in their own
+// tree walk, instead of waiting for the widest child loop of the wavefront at every level.
 template <int FORM, int SEM, int KT, bool CNT>
 __device__ void traverse(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv,
                          V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff, uint32_t minDepth,
@@ -320,27 +289,105 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KParams& kp, cons
     if constexpr (CNT) ct.far_ += 1;
     return;
   }
-  expand<FORM, SEM, KT, CNT>(acc, kp, tv, 0, h0, g0, d0, T(1), q, cutoff, minDepth, st, 0, tid, ct);
+  // current node = root
+  uint32_t node = 0, first = h0.x, nc = hdr_count(h0.y), k = 0, mask = 0;
   int lvl = 0;
-  while (lvl >= 0) {
-    const uint32_t mask = st.mask[lvl][tid];
-    if (mask == 0) {
-      --lvl;
-      continue;
+  bool blendNode = minDepth == 0;
+  T m = T(1), distP = blendNode ? fabs(d0 - (T)g0.w * cutoff) : T(-1), shareSum = T(0);
+  float Ai = __uint_as_float(h0.z);
+  bool active = true;
+  while (active) {
+    // --- pick X: the next child of the current node, or the next pending node to expand
+    const bool testing = k < nc;
+    uint32_t x;
+    T mp, dp;
+    bool ownContrib = false;
+    uint32_t ownNode = node;
+    float ownA = Ai;
+    T ownF = T(0);
+    if (testing) {
+      x = first + k;
+      mp = m;
+      dp = blendNode ? distP : T(-1);
+    } else {
+      ownContrib = blendNode && shareSum != T(0);
+      ownF = m * shareSum;
+      if constexpr (CNT) ct.nearN += blendNode ? 1u : 0u;
+      st.first[lvl][tid] = first;
+      st.mask[lvl][tid] = (uint8_t)mask;
+      st.mul[lvl][tid] = m;
+      st.dist[lvl][tid] = blendNode ? distP : T(-1);
+      int L = lvl;
+      uint32_t pm = mask;
+      while (pm == 0u && L > 0) {
+        --L;
+        pm = st.mask[L][tid];
+      }
+      if (pm == 0u) {
+        active = false;
+        x = 0;
+        mp = T(0);
+        dp = T(-1);
+      } else {
+        st.mask[L][tid] = (uint8_t)(pm & (pm - 1u));
+        x = st.first[L][tid] + (uint32_t)__builtin_ctz(pm);
+        mp = st.mul[L][tid];
+        dp = st.dist[L][tid];
+        lvl = L + 1;
+      }
     }
-    st.mask[lvl][tid] = mask & (mask - 1u);
-    const uint32_t c = st.first[lvl][tid] + (uint32_t)__builtin_ctz(mask);
-    const float4 gc = tv.geo[c];
-    const uint4 hc = tv.hdr[c];
-    const T dcq = dist3(q, gc);
-    T mc = st.mul[lvl][tid];
-    const T dp = st.dist[lvl][tid];
-    if (dp >= T(0)) {
-      const T dc = fmax(dcq - (T)gc.w * cutoff, T(0));
-      mc *= blend(dp / (dp + dc));
+    // --- X's geometry and blend factor against its parent (CPUUtils.cpp:489-497)
+    bool contrib = ownContrib;
+    uint32_t tgt = ownNode;
+    float At = ownA;
+    T f = ownF;
+    if (active) {
+      const float4 gx = tv.geo[x];
+      const uint4 hx = tv.hdr[x];
+      const T dxq = dist3(q, gx);
+      const T rc = (T)gx.w * cutoff;
+      T beta = T(1);
+      if (dp >= T(0)) {
+        const T dc = fmax(dxq - rc, T(0));
+        beta = blend(dp / (dp + dc));
+      }
+      const T mx = mp * beta;
+      const float Ax = __uint_as_float(hx.z);
+      if (testing) {
+        ++k;
+        if (blendNode) {
+          const T share = SEM == MLOD_SEM_REF_CPU ? T(1) / (T)nc : (T)Ax / (T)Ai;
+          shareSum += (T(1) - beta) * share;
+        }
+        if constexpr (CNT) ct.tested += 1;
+        const bool leaf = hdr_leaf(hx.y);
+        const bool single = leaf && hdr_count(hx.y) == 1u;
+        const bool far = !leaf && hdr_depth(hx.y) >= minDepth && dxq > rc;
+        if (leaf && !single) add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, hx, q, mx, ct);  // depth-12 multi-point leaf
+        if (!leaf && !far) mask |= 1u << (k - 1u);
+        if constexpr (CNT) {
+          ct.far_ += far ? 1u : 0u;
+          ct.leafPts += single ? 1u : 0u;
+        }
+        contrib = far || single;
+        tgt = x;
+        At = Ax;
+        f = mx;
+      } else {
+        // X becomes the current node
+        node = x;
+        first = hx.x;
+        nc = hdr_count(hx.y);
+        k = 0;
+        mask = 0;
+        blendNode = hdr_depth(hx.y) >= minDepth;
+        distP = fabs(dxq - rc);
+        m = mx;
+        shareSum = T(0);
+        Ai = Ax;
+      }
     }
-    expand<FORM, SEM, KT, CNT>(acc, kp, tv, c, hc, gc, dcq, mc, q, cutoff, minDepth, st, lvl + 1, tid, ct);
-    ++lvl;
+    if (contrib) add_node<FORM, KT>(acc, kp, tv, tgt, At, q, f);
   }
 }
 
@@ -360,7 +407,12 @@ __device__ __forceinline__ T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b
 
 // Sums in frame `o` (origin of the accumulated positions): q is the query in that frame.
 // Returns the projection in that frame (NaN guard and step clamp applied).
-template <class T, class TF, int SEM>
+// STABLE (query frame only, q = 0): the sphere point c + r (q - c)/|q - c| is computed as
+// tau * u/|u| with tau = -2 u0 / (|u| + sqrt(|u|^2 - 4 u4 u0)), the small root of
+// u0 + tau |u| + u4 tau^2 = 0 along the line through q and the centre.  It is the same point
+// (|c| - r = 2 u0 / (|u| + sqrt(|u|^2 - 4 u4 u0)) up to sign) without the cancellation of c and
+// r near planar fits (|u4| -> 0, |c| ~ 1/|u4|), and tends to the plane projection as u4 -> 0.
+template <class T, class TF, int SEM, bool STABLE = false>
 __device__ __forceinline__ void fit(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T> q, T stepMax, V3<T>& outP, V3<T>& outN) {
   const T A = spn - dot3(sp, V3<T>{sn.x / sw, sn.y / sw, sn.z / sw});
   const T B = spp - dot3(sp, V3<T>{sp.x / sw, sp.y / sw, sp.z / sw});
@@ -375,6 +427,18 @@ __device__ __forceinline__ void fit(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T
     const T lambda = (u0 - dot3(p, nn)) / dot3(nn, nn);
     p = V3<T>{p.x + lambda * nn.x, p.y + lambda * nn.y, p.z + lambda * nn.z};
     nr = nrmz<T, SEM>(sn);
+  } else if constexpr (STABLE) {
+    const T un2 = dot3(u, u);
+    const T un = fsqrt(un2);
+    const T disc = un2 - T(4) * u4 * u0;
+    if (disc < T(0)) {  // r^2 < 0: the reference clamps r to 0, i.e. the centre
+      const T m2 = T(-2) * u4;
+      p = V3<T>{u.x / m2, u.y / m2, u.z / m2};
+    } else {
+      const T s = (T(-2) * u0 / (un + fsqrt(disc))) / un;
+      p = V3<T>{s * u.x, s * u.y, s * u.z};
+    }
+    nr = nrmz<T, SEM>(V3<T>{u.x + two_u4 * p.x, u.y + two_u4 * p.y, u.z + two_u4 * p.z});
   } else {
     const T m2 = T(-2) * u4;
     const V3<T> c{u.x / m2, u.y / m2, u.z / m2};
@@ -412,7 +476,7 @@ __device__ __forceinline__ void fit_step(const typename FormT<FORM>::Acc& acc, V
   } else if constexpr (FORM == 1) {
     // query frame: the query is the origin
     V3<double> p, n;
-    fit<double, double, SEM>(acc.w.val(), V3<double>{acc.px.val(), acc.py.val(), acc.pz.val()},
+    fit<double, double, SEM, true>(acc.w.val(), V3<double>{acc.px.val(), acc.py.val(), acc.pz.val()},
                              V3<double>{acc.nx.val(), acc.ny.val(), acc.nz.val()}, acc.pn.val(), acc.pp.val(),
                              V3<double>{0, 0, 0}, (double)stepMax, p, n);
     qd = V3<double>{(double)qf.x + p.x, (double)qf.y + p.y, (double)qf.z + p.z};
@@ -436,7 +500,7 @@ __global__ __launch_bounds__(PBS) void k_project(TreeView tv, KParams kp, float4
                                                  unsigned long long* __restrict__ counters) {
   using T = typename FormT<FORM>::T;
   __shared__ uint32_t sFirst[kLevels][PBS];
-  __shared__ uint32_t sMask[kLevels][PBS];
+  __shared__ uint8_t sMask[kLevels][PBS];
   __shared__ T sMul[kLevels][PBS];
   __shared__ T sDist[kLevels][PBS];
   const int tid = threadIdx.x;

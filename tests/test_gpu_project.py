@@ -190,3 +190,78 @@ def test_state_errors():
     ap.project(0, None, None, 1)  # empty batch is a no-op (Fastapss.cu:441-445)
     with pytest.raises(MlodError):
         ap.project(11, None, None, 1)  # more than uploaded
+
+
+NB_GOLDEN = [n for n in golden_names("proj_") if "knn" in n or "ball" in n]
+
+
+def _ball_counts(P, Q, r):
+    P = P.astype(np.float64)
+    return np.array([(np.sum((P - q) ** 2, 1) <= r * r).sum() for q in np.asarray(Q, np.float64)])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32c", "fp64"])
+@pytest.mark.parametrize("name", NB_GOLDEN)
+def test_knn_ball_match_reference_golden(oracle_built, name, precision):
+    """kNN / ball APSS (the reference's CPU-only projectAPSSCPU) against its golden vectors
+    and the FP64 oracle.  Degenerate balls (< 4 points) are rounding noise in every form and
+    are left out of the error; a single-point ball's NaN fit must reproduce in FP32."""
+    from gpu_util import gpu_project, rel_err
+    g = load_golden(name)
+    tn = str(g["tree"])
+    t = load_golden(f"tree_{tn}.npz")
+    print("case", name, precision, flush=True)
+    tree = tree_for(tn, t["points"], t["normals"])
+    mode = MODE_NAMES[int(g["mode"])]
+    kern = golden_kernel(g)
+    op, on, _ = gpu_project(tree, g["queries"], kern, semantics="ref_cpu", precision=precision,
+                            n_iter=int(g["n_iter"]), mode=mode, knn=int(g["knn"]), ball_radius=float(g["ball_radius"]))
+    diag = 2 * float(t["radius"][0])
+    ref = g["out_points"]
+    ok = np.isfinite(ref).all(1) & np.isfinite(op).all(1)
+    if mode == "ball":
+        r = 3 * float(g["kernel_sigmas"][0])
+        cnt = _ball_counts(t["points"], np.where(np.isfinite(ref), ref, 0), r)
+        ok &= cnt >= 4
+        if precision == "fp32":
+            one = ~np.isfinite(ref).all(1) & (_ball_counts(t["points"], g["queries"], r) == 1)
+            assert np.all(~np.isfinite(op[one]).all(1))
+    assert ok.mean() > 0.9
+    e = rel_err(op[ok], ref[ok], diag)
+    assert np.median(e) < 1e-7 and e.max() < TOL_REF, e.max()
+    o = oracle_built.OracleTree(t["points"], t["normals"])
+    prm = oracle_built.make_params(semantics="ref_cpu", mode=mode, knn=int(g["knn"]), ball_radius=float(g["ball_radius"]))
+    o64, _ = o.project(g["queries"], oracle_built.make_kernel(**kern), prm, n_iter=int(g["n_iter"]), precision=1)
+    ok &= np.isfinite(o64).all(1)
+    e = rel_err(op[ok], o64[ok].astype(np.float32), diag)
+    assert e.max() < TOL_ORACLE[precision], e.max()
+
+
+@pytest.mark.parametrize("k", [5, 20, 32, 50])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c"])
+def test_knn_matches_oracle(oracle_built, k, precision):
+    from gpu_util import gpu_project, rel_err
+    p, n = ds.torus(30000, seed=51)
+    tree = tree_for("torus30k", p, n)
+    R = ds.bbox_radius(p)
+    kern = KERNELS["mg4"](R)
+    q = ds.jitter_queries(p[:2000], 0.01, seed=52)
+    op, _, _ = gpu_project(tree, q, kern, semantics="ref_cpu", precision=precision, n_iter=2, mode="knn", knn=k)
+    o = oracle_built.OracleTree(p, n)
+    o64, _ = o.project(q, oracle_built.make_kernel(**kern), oracle_built.make_params(mode="knn", knn=k), n_iter=2,
+                       precision=1)
+    ok = np.isfinite(o64).all(1)
+    assert ok.mean() > 0.99
+    tol = TOL_ORACLE[precision] if k >= 20 else 1e-4  # few neighbours: ill-conditioned fits
+    e = rel_err(op[ok], o64[ok].astype(np.float32), 2 * R)
+    assert np.median(e) < 1e-7 and e.max() < tol, e.max()
+
+
+def test_ball_empty_returns_start():
+    """A query whose ball is empty comes back unchanged with a zero normal (CPUUtils.cpp:711-716)."""
+    from gpu_util import gpu_project
+    p, n = ds.sphere(5000, seed=53)
+    tree = tree_for("sphere5k_b", p, n)
+    q = np.array([[3.0, 0, 0], [0, 2.5, 0]], np.float32)
+    op, on, _ = gpu_project(tree, q, dict(type=2, sigmas=[0.01, 0.02]), mode="ball", ball_radius=-1.0, n_iter=3)
+    assert np.array_equal(op, q) and np.all(on == 0)
