@@ -133,6 +133,37 @@ int mlod_tree_dump(const mlod_tree* t, int32_t* depth, float* bb6, float* center
  * out4 receives the four totals. */
 int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out4, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Serving.  No counterpart in the reference (its harness projects one batch at a time with
+ * blocking calls, scene.cpp:916-966).  A server keeps registered octrees resident, takes
+ * projection requests from any thread, packs pending requests oldest-first into batches of at
+ * most max_batch_queries queries and keeps up to `inflight` batches on the device at once
+ * (continuous batching).  Each query gives the same result alone or in any batch.
+ * Registered trees must outlive the server; request buffers must stay valid until the
+ * request's completion is polled. */
+typedef struct mlod_server mlod_server;
+typedef struct {
+  uint64_t ticket;
+  double latency_ms;   /* submit -> results copied to the caller's buffers */
+  double queue_ms;     /* submit -> its batch launched */
+  double completed_s;  /* completion time since server creation */
+} mlod_server_done;
+
+int mlod_server_create(uint32_t max_batch_queries, uint32_t inflight, int32_t semantics, int32_t precision,
+                       mlod_server** out);
+int mlod_server_destroy(mlod_server* s);
+int mlod_server_add_tree(mlod_server* s, mlod_tree* t, uint32_t* tree_id);
+/* n_iter projection steps of m queries (host, m x 3) on tree tree_id with kernel k; results go
+ * to out_pts / out_nmls (host, m x 3, either may be NULL).  step_max < 0 => the tree's radius.
+ * Multi-Gaussian kernels take at most 8 sigmas here. */
+int mlod_server_submit(mlod_server* s, uint32_t tree_id, const mlod_kernel* k, float cutoff, uint32_t min_depth,
+                       uint32_t n_iter, float step_max, const float* queries, uint32_t m, float* out_pts,
+                       float* out_nmls, uint64_t* ticket);
+/* Completed requests since the last poll (waits up to timeout_s seconds for one). */
+int mlod_server_poll(mlod_server* s, mlod_server_done* out, uint32_t cap, uint32_t* n_out, double timeout_s);
+/* batches launched, requests completed, queries launched, requests pending */
+int mlod_server_stats(mlod_server* s, uint64_t* out4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,8 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+std::string g_last_error() { return g_err; }
+
 int hip_fail(hipError_t e, const char* what, const char* file, int line) {
   g_err = std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")";
   return e == hipErrorOutOfMemory ? MLOD_ERR_OOM : MLOD_ERR_HIP;

@@ -63,6 +63,34 @@ struct KParams {
   double bd[MLOD_MAX_SIGMAS];  // FP64: 1 / s^3
 };
 
+// Kernel coefficients of one serving request (same fields and meaning as KParams, at most 8
+// sigmas), read per lane from the request table.
+constexpr int kServeMaxSigmas = 8;
+struct WCoef {
+  int32_t type;
+  int32_t ns;
+  int32_t eint;
+  float c;
+  float cp;
+  float e;
+  float a[kServeMaxSigmas];
+  float b[kServeMaxSigmas];
+};
+
+// One request of a serving batch: its tree, kernel, parameters and slot range.
+struct ReqDesc {
+  TreeView tv;
+  WCoef w;
+  float cutoff;
+  float stepMax;
+  uint32_t minDepth;
+  uint32_t nIter;
+  uint32_t slot0;   // first slot of the request in the batch
+  uint32_t count;   // queries
+  float org[3];     // tree box origin and Morton scale
+  float scale[3];
+};
+
 struct QueryState {
   float4* q;         // sorted query positions (in place across project calls)
   float4* qn;        // normals of the last step
@@ -115,6 +143,7 @@ namespace mlod {
 
 // error plumbing (api.cpp)
 int fail(int code, const std::string& msg);
+std::string g_last_error();  // the calling thread's last message
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
 #define MLOD_HIP(expr)                                                         \
@@ -133,6 +162,12 @@ int dump_tree(const mlod_tree* t, int32_t* depth, float* bb6, float* center3, fl
 KParams make_kparams(const mlod_kernel& k);
 int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s);
 int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream_t s);
+
+// serving batch (project.hip): requests [r0, r1) of `reqs` with kernel type kt, slots sorted so
+// that slot i belongs to request slotReq[i]
+int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_slotReq, float4* d_q, float4* d_qn, uint32_t begin,
+                         uint32_t end, int kt, int semantics, int precision, hipStream_t s);
+WCoef make_wcoef(const mlod_kernel& k);
 
 // neighbors.hip
 int launch_neighbors(mlod_tree* t, uint32_t n, uint32_t n_iter, hipStream_t s);

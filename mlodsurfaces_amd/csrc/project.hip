@@ -41,8 +41,8 @@ constexpr int kLevels = kMaxDepth;  // internal nodes have depth 0..11
 
 // ----------------------------------------------------------------------------- weights
 // FP32 weights (Kernel.cpp:64-83 / GPUUtils.cu:243-257) from the squared distance.
-template <int KT>
-__device__ __forceinline__ float weight32(const KParams& k, float d2) {
+template <int KT, class KP>
+__device__ __forceinline__ float weight32(const KP& k, float d2) {
   if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
     float w = 0.f;
     if (k.ns <= 8) {
@@ -138,8 +138,8 @@ struct Counters {
 
 // Contribution of one point (leaf member) with multiplier f, weight at the point's mean
 // position (a p) / a (Fastapss.cu:193; CPUUtils.cpp:455).
-template <int FORM, int KT>
-__device__ __forceinline__ void add_point(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv, uint32_t j,
+template <int FORM, int KT, class KP>
+__device__ __forceinline__ void add_point(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv, uint32_t j,
                                           V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f) {
   const float4 P = tv.pts[j], N = tv.nrm[j];
   if constexpr (FORM == 0) {
@@ -187,8 +187,8 @@ __device__ __forceinline__ void add_point(typename FormT<FORM>::Acc& acc, const 
 }
 
 // Contribution of an internal node's moments with multiplier f, weight at its mean point.
-template <int FORM, int KT>
-__device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv, uint32_t c,
+template <int FORM, int KT, class KP>
+__device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv, uint32_t c,
                                          float Af, V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f) {
   if constexpr (FORM == 0) {
     const float4 m0 = tv.momP[2 * (size_t)c], m1 = tv.momP[2 * (size_t)c + 1];
@@ -235,8 +235,8 @@ __device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const K
   }
 }
 
-template <int FORM, int SEM, int KT, bool CNT>
-__device__ __forceinline__ void add_leaf(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv, uint4 h,
+template <int FORM, int SEM, int KT, bool CNT, class KP>
+__device__ __forceinline__ void add_leaf(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv, uint4 h,
                                          V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f, Counters& ct) {
   const uint32_t first = h.x, cnt = hdr_count(h.y);
   if constexpr (SEM == MLOD_SEM_REF_CPU) {
@@ -272,8 +272,8 @@ struct LevelState {
 // near) or makes it the new current node, and ends in one shared add_node.  Lanes of a
 // wavefront therefore advance by one node per iteration whatever their position in their own
 // tree walk, instead of waiting for the widest child loop of the wavefront at every level.
-template <int FORM, int SEM, int KT, bool CNT>
-__device__ void traverse(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv,
+template <int FORM, int SEM, int KT, bool CNT, class KP>
+__device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv,
                          V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff, uint32_t minDepth,
                          LevelState<typename FormT<FORM>::T> st, int tid, Counters& ct) {
   using T = typename FormT<FORM>::T;
@@ -581,6 +581,54 @@ __global__ __launch_bounds__(PBS) void k_global(TreeView tv, KParams kp, float4*
   }
 }
 
+// Serving batch: lane i projects slot i of the batch with the tree, kernel and parameters of
+// its request (a per-lane read of the request table); per-lane work and results are those of
+// k_project, so a query gives the same bits alone or in any batch.
+template <int FORM, int SEM, int KT>
+__global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict__ reqs, const uint32_t* __restrict__ slotReq,
+                                                       float4* __restrict__ q, float4* __restrict__ qn, uint32_t begin,
+                                                       uint32_t end) {
+  using T = typename FormT<FORM>::T;
+  __shared__ uint32_t sFirst[kLevels][PBS];
+  __shared__ uint8_t sMask[kLevels][PBS];
+  __shared__ T sMul[kLevels][PBS];
+  __shared__ T sDist[kLevels][PBS];
+  const int tid = threadIdx.x;
+  const uint32_t i = begin + blockIdx.x * PBS + tid;
+  if (i >= end) return;
+  const ReqDesc& R = reqs[slotReq[i]];
+  const TreeView tv = R.tv;
+  const WCoef w = R.w;
+  const float cutoff = R.cutoff, stepMax = R.stepMax;
+  const uint32_t minDepth = R.minDepth, nIter = R.nIter;
+  const LevelState<T> st{sFirst, sMask, sMul, sDist};
+  const float4 q4 = q[i];
+  V3<double> qd{q4.x, q4.y, q4.z};
+  float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
+  Counters ct{0, 0, 0, 0};
+  for (uint32_t it = 0; it < nIter; ++it) {
+    typename FormT<FORM>::Acc acc;
+    acc.init();
+    const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
+    const V3<T> qt{qf.x, qf.y, qf.z};
+    traverse<FORM, SEM, KT, false>(acc, w, tv, qt, (T)cutoff, minDepth, st, tid, ct);
+    fit_step<FORM, SEM>(acc, qd, qf, stepMax, nout);
+  }
+  q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
+  qn[i] = nout;
+}
+
+template <int FORM, int SEM>
+void launch_batch_kt(const ReqDesc* r, const uint32_t* sr, float4* q, float4* qn, uint32_t b, uint32_t e, int kt,
+                     hipStream_t s) {
+  const uint32_t grid = (e - b + PBS - 1) / PBS;
+  switch (kt) {
+    case MLOD_KERNEL_GAUSSIAN: k_project_batch<FORM, SEM, MLOD_KERNEL_GAUSSIAN><<<grid, PBS, 0, s>>>(r, sr, q, qn, b, e); break;
+    case MLOD_KERNEL_SINGULAR: k_project_batch<FORM, SEM, MLOD_KERNEL_SINGULAR><<<grid, PBS, 0, s>>>(r, sr, q, qn, b, e); break;
+    default: k_project_batch<FORM, SEM, MLOD_KERNEL_GAUSSIAN_MULTIPLE><<<grid, PBS, 0, s>>>(r, sr, q, qn, b, e); break;
+  }
+}
+
 template <int FORM, int SEM, int KT, bool CNT>
 void launch_one(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax,
                 unsigned long long* counters, hipStream_t s) {
@@ -640,6 +688,39 @@ KParams make_kparams(const mlod_kernel& k) {
     p.ad[0] = -1.0 / (2.0 * sg * sg);
   }
   return p;
+}
+
+WCoef make_wcoef(const mlod_kernel& k) {
+  const KParams p = make_kparams(k);
+  WCoef w{};
+  w.type = p.type;
+  w.ns = p.ns;
+  w.eint = p.eint;
+  w.c = p.c;
+  w.cp = p.cp;
+  w.e = p.e;
+  for (int s = 0; s < kServeMaxSigmas; ++s) {
+    w.a[s] = p.a[s];
+    w.b[s] = p.b[s];
+  }
+  return w;
+}
+
+int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_slotReq, float4* d_q, float4* d_qn, uint32_t begin,
+                         uint32_t end, int kt, int semantics, int precision, hipStream_t s) {
+  if (end <= begin) return MLOD_OK;
+  const bool cpu = semantics == MLOD_SEM_REF_CPU;
+  if (precision == MLOD_PREC_FP32C) {
+    if (cpu) launch_batch_kt<1, MLOD_SEM_REF_CPU>(d_reqs, d_slotReq, d_q, d_qn, begin, end, kt, s);
+    else launch_batch_kt<1, MLOD_SEM_REF_GPU>(d_reqs, d_slotReq, d_q, d_qn, begin, end, kt, s);
+  } else if (precision == MLOD_PREC_FP32) {
+    if (cpu) launch_batch_kt<0, MLOD_SEM_REF_CPU>(d_reqs, d_slotReq, d_q, d_qn, begin, end, kt, s);
+    else launch_batch_kt<0, MLOD_SEM_REF_GPU>(d_reqs, d_slotReq, d_q, d_qn, begin, end, kt, s);
+  } else {
+    return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
+  }
+  LAUNCH_CHECK();
+  return MLOD_OK;
 }
 
 int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s) {

@@ -1,0 +1,465 @@
+// serve.hip — continuous-batching projection server over resident octrees.
+//
+// The reference has no server: its harness projects one query batch at a time with blocking
+// calls (scene.cpp:916-966).  Here clients submit projection requests (tree, kernel,
+// parameters, queries, n_iter) from any thread; a worker thread packs the pending requests,
+// oldest first, into batches bounded by a slot budget (device memory per batch is fixed at
+// creation), and keeps up to `inflight` batches on separate streams, so requests that arrive
+// while others are on the GPU join the next batch as soon as a batch slot frees, and the
+// batches' copies and kernels overlap on the device.
+//
+// One batch on its stream: pinned host staging -> device copy; per-slot Morton key in the
+// request's tree box with the request index above it; radix sort (so each wavefront walks one
+// request's neighbourhood); gather; one projection launch per kernel type over that type's
+// contiguous slot range (k_project_batch: per-lane request table, the same per-query code and
+// bits as a standalone projection); scatter back to submission order; device -> pinned copy;
+// completion event.  The worker copies each request's results to the caller's buffers when
+// the event has fired and stamps the completion time.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "mlod_internal.h"
+
+namespace mlod {
+namespace {
+
+using clk = std::chrono::steady_clock;
+constexpr int BS = 256;
+constexpr uint32_t kMaxReqPerBatch = 1024;  // request index fits 10 bits above the 30-bit Morton code
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// key = request index << 30 | Morton code of the query in its tree's box
+__global__ __launch_bounds__(BS) void k_batch_keys(const float* __restrict__ in, uint32_t slots,
+                                                   const ReqDesc* __restrict__ reqs, uint32_t nReq,
+                                                   uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= slots) return;
+  uint32_t lo = 0, hi = nReq;  // last request with slot0 <= i
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (reqs[mid].slot0 <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const ReqDesc& R = reqs[lo];
+  uint32_t c[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float u = (in[3 * (uint64_t)i + a] - R.org[a]) * R.scale[a];
+    u = fminf(fmaxf(u, 0.f), 1023.f);
+    c[a] = (uint32_t)u;
+  }
+  key[i] = ((uint64_t)lo << 30) | (spread10(c[0]) | (spread10(c[1]) << 1) | (spread10(c[2]) << 2));
+  idx[i] = i;
+}
+
+__global__ __launch_bounds__(BS) void k_batch_gather(const float* __restrict__ in, const uint64_t* __restrict__ key,
+                                                     const uint32_t* __restrict__ perm, uint32_t slots,
+                                                     float4* __restrict__ q, float4* __restrict__ qn,
+                                                     uint32_t* __restrict__ slotReq) {
+  const uint32_t j = blockIdx.x * BS + threadIdx.x;
+  if (j >= slots) return;
+  const uint32_t s = perm[j];
+  q[j] = make_float4(in[3 * (uint64_t)s], in[3 * (uint64_t)s + 1], in[3 * (uint64_t)s + 2], 0.f);
+  qn[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+  slotReq[j] = (uint32_t)(key[j] >> 30);
+}
+
+// out = [slots x 3 positions | slots x 3 normals] in submission order
+__global__ __launch_bounds__(BS) void k_batch_scatter(const float4* __restrict__ q, const float4* __restrict__ qn,
+                                                      const uint32_t* __restrict__ perm, uint32_t slots, uint32_t cap,
+                                                      float* __restrict__ out) {
+  const uint32_t j = blockIdx.x * BS + threadIdx.x;
+  if (j >= slots) return;
+  const uint32_t d = perm[j];
+  const float4 p = q[j], n = qn[j];
+  out[3 * (uint64_t)d] = p.x;
+  out[3 * (uint64_t)d + 1] = p.y;
+  out[3 * (uint64_t)d + 2] = p.z;
+  out[3 * ((uint64_t)cap + d)] = n.x;
+  out[3 * ((uint64_t)cap + d) + 1] = n.y;
+  out[3 * ((uint64_t)cap + d) + 2] = n.z;
+}
+
+struct SReq {
+  uint64_t ticket;
+  uint32_t tree;
+  mlod_kernel kernel;
+  float cutoff;
+  uint32_t minDepth;
+  uint32_t nIter;
+  float stepMax;
+  const float* in;
+  float* outP;
+  float* outN;
+  uint32_t m;
+  clk::time_point submitted;
+};
+
+struct BatchSlot {
+  hipStream_t s = nullptr;
+  hipEvent_t ev = nullptr;
+  float* hIn = nullptr;
+  float* hOut = nullptr;
+  ReqDesc* hReq = nullptr;
+  float* dIn = nullptr;
+  float* dOut = nullptr;
+  float4* dQ = nullptr;
+  float4* dQn = nullptr;
+  uint64_t* dKey = nullptr;
+  uint64_t* dKey2 = nullptr;
+  uint32_t* dIdx = nullptr;
+  uint32_t* dPerm = nullptr;
+  uint32_t* dSlotReq = nullptr;
+  ReqDesc* dReq = nullptr;
+  void* sortTmp = nullptr;
+  size_t sortBytes = 0;
+  std::vector<SReq> reqs;
+  uint32_t slots = 0;
+  bool busy = false;
+  clk::time_point launched;
+};
+
+}  // namespace
+}  // namespace mlod
+
+struct mlod_server {
+  uint32_t cap = 0;
+  int semantics = MLOD_SEM_REF_GPU;
+  int precision = MLOD_PREC_FP32C;
+  int device = 0;
+  std::vector<mlod_tree*> trees;
+  std::vector<mlod::BatchSlot> slots;
+  std::deque<mlod::SReq> pending;
+  std::vector<mlod_server_done> done;
+  std::mutex mu;
+  std::condition_variable cv, cvDone;
+  std::thread worker;
+  std::atomic<bool> stop{false};
+  std::atomic<int> error{0};
+  std::string errmsg;
+  uint64_t nextTicket = 1;
+  uint64_t batches = 0, queries = 0, requests = 0;
+  mlod::clk::time_point t0 = mlod::clk::now();
+};
+
+namespace mlod {
+namespace {
+
+template <class T>
+hipError_t dmalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+template <class T>
+hipError_t hmalloc(T** p, size_t n) {
+  return hipHostMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault);
+}
+
+void free_slot(BatchSlot& b) {
+  if (b.s) (void)hipStreamSynchronize(b.s);
+  void* d[] = {b.dIn, b.dOut, b.dQ, b.dQn, b.dKey, b.dKey2, b.dIdx, b.dPerm, b.dSlotReq, b.dReq, b.sortTmp};
+  for (void* p : d)
+    if (p) (void)hipFree(p);
+  void* h[] = {b.hIn, b.hOut, b.hReq};
+  for (void* p : h)
+    if (p) (void)hipHostFree(p);
+  if (b.ev) (void)hipEventDestroy(b.ev);
+  if (b.s) (void)hipStreamDestroy(b.s);
+  b = BatchSlot{};
+}
+
+hipError_t alloc_slot(BatchSlot& b, uint32_t cap) {
+  hipError_t e;
+  if ((e = hipStreamCreate(&b.s)) || (e = hipEventCreateWithFlags(&b.ev, hipEventDisableTiming))) return e;
+  if ((e = hmalloc(&b.hIn, 3 * (size_t)cap)) || (e = hmalloc(&b.hOut, 6 * (size_t)cap)) ||
+      (e = hmalloc(&b.hReq, kMaxReqPerBatch)))
+    return e;
+  if ((e = dmalloc(&b.dIn, 3 * (size_t)cap)) || (e = dmalloc(&b.dOut, 6 * (size_t)cap)) || (e = dmalloc(&b.dQ, cap)) ||
+      (e = dmalloc(&b.dQn, cap)) || (e = dmalloc(&b.dKey, cap)) || (e = dmalloc(&b.dKey2, cap)) ||
+      (e = dmalloc(&b.dIdx, cap)) || (e = dmalloc(&b.dPerm, cap)) || (e = dmalloc(&b.dSlotReq, cap)) ||
+      (e = dmalloc(&b.dReq, kMaxReqPerBatch)))
+    return e;
+  if ((e = rocprim::radix_sort_pairs(nullptr, b.sortBytes, b.dKey, b.dKey2, b.dIdx, b.dPerm, cap, 0, 40, b.s))) return e;
+  return hipMalloc(&b.sortTmp, b.sortBytes + 16);
+}
+
+// Packs pending requests (oldest first) into slot b and launches its work.  Caller holds mu.
+int launch_batch(mlod_server* S, BatchSlot& b) {
+  b.reqs.clear();
+  uint32_t slots = 0;
+  while (!S->pending.empty() && b.reqs.size() < kMaxReqPerBatch && slots + S->pending.front().m <= S->cap) {
+    slots += S->pending.front().m;
+    b.reqs.push_back(S->pending.front());
+    S->pending.pop_front();
+  }
+  if (b.reqs.empty()) return MLOD_OK;
+  // group by kernel type: each type is one contiguous slot range and one launch
+  std::stable_sort(b.reqs.begin(), b.reqs.end(),
+                   [](const SReq& x, const SReq& y) { return x.kernel.type < y.kernel.type; });
+  uint32_t off = 0, typeBeg[3] = {0, 0, 0}, typeEnd[3] = {0, 0, 0};
+  bool seen[3] = {false, false, false};
+  for (size_t r = 0; r < b.reqs.size(); ++r) {
+    const SReq& q = b.reqs[r];
+    const mlod_tree* t = S->trees[q.tree];
+    ReqDesc& d = b.hReq[r];
+    d.tv = t->view();
+    d.w = make_wcoef(q.kernel);
+    d.cutoff = q.cutoff;
+    d.stepMax = q.stepMax < 0.f ? t->radius : q.stepMax;
+    d.minDepth = q.minDepth;
+    d.nIter = q.nIter;
+    d.slot0 = off;
+    d.count = q.m;
+    for (int a = 0; a < 3; ++a) {
+      const float ext = t->BB[a] - t->bb[a];
+      d.org[a] = t->bb[a];
+      d.scale[a] = ext > 0.f ? 1024.f / ext : 0.f;
+    }
+    std::memcpy(b.hIn + 3 * (size_t)off, q.in, (size_t)q.m * 3 * sizeof(float));
+    const int kt = q.kernel.type;
+    if (!seen[kt]) {
+      typeBeg[kt] = off;
+      seen[kt] = true;
+    }
+    off += q.m;
+    typeEnd[kt] = off;
+  }
+  b.slots = slots;
+  const uint32_t nReq = (uint32_t)b.reqs.size();
+  int reqBits = 1;
+  while ((1u << reqBits) < nReq) ++reqBits;
+  hipError_t e = hipMemcpyAsync(b.dIn, b.hIn, (size_t)slots * 3 * sizeof(float), hipMemcpyHostToDevice, b.s);
+  if (!e) e = hipMemcpyAsync(b.dReq, b.hReq, nReq * sizeof(ReqDesc), hipMemcpyHostToDevice, b.s);
+  const uint32_t grid = (slots + BS - 1) / BS;
+  if (!e) {
+    k_batch_keys<<<grid, BS, 0, b.s>>>(b.dIn, slots, b.dReq, nReq, b.dKey, b.dIdx);
+    e = hipGetLastError();
+  }
+  if (!e) e = rocprim::radix_sort_pairs(b.sortTmp, b.sortBytes, b.dKey, b.dKey2, b.dIdx, b.dPerm, slots, 0, 30 + reqBits, b.s);
+  if (!e) {
+    k_batch_gather<<<grid, BS, 0, b.s>>>(b.dIn, b.dKey2, b.dPerm, slots, b.dQ, b.dQn, b.dSlotReq);
+    e = hipGetLastError();
+  }
+  if (e) return hip_fail(e, "serving batch", __FILE__, __LINE__);
+  for (int kt = 0; kt < 3; ++kt) {
+    if (!seen[kt]) continue;
+    const int r = launch_project_batch(b.dReq, b.dSlotReq, b.dQ, b.dQn, typeBeg[kt], typeEnd[kt], kt, S->semantics,
+                                       S->precision, b.s);
+    if (r) return r;
+  }
+  k_batch_scatter<<<grid, BS, 0, b.s>>>(b.dQ, b.dQn, b.dPerm, slots, S->cap, b.dOut);
+  e = hipGetLastError();
+  if (!e) e = hipMemcpyAsync(b.hOut, b.dOut, (size_t)slots * 3 * sizeof(float), hipMemcpyDeviceToHost, b.s);
+  if (!e)
+    e = hipMemcpyAsync(b.hOut + 3 * (size_t)S->cap, b.dOut + 3 * (size_t)S->cap, (size_t)slots * 3 * sizeof(float),
+                       hipMemcpyDeviceToHost, b.s);
+  if (!e) e = hipEventRecord(b.ev, b.s);
+  if (e) return hip_fail(e, "serving batch", __FILE__, __LINE__);
+  b.busy = true;
+  b.launched = clk::now();
+  S->batches += 1;
+  S->queries += slots;
+  return MLOD_OK;
+}
+
+// Copies results of a finished batch to the callers.  Caller holds mu.
+void complete_batch(mlod_server* S, BatchSlot& b) {
+  const auto now = clk::now();
+  for (size_t r = 0; r < b.reqs.size(); ++r) {
+    const SReq& q = b.reqs[r];
+    const uint32_t off = b.hReq[r].slot0;
+    if (q.outP) std::memcpy(q.outP, b.hOut + 3 * (size_t)off, (size_t)q.m * 3 * sizeof(float));
+    if (q.outN) std::memcpy(q.outN, b.hOut + 3 * ((size_t)S->cap + off), (size_t)q.m * 3 * sizeof(float));
+    mlod_server_done d;
+    d.ticket = q.ticket;
+    d.latency_ms = std::chrono::duration<double, std::milli>(clk::now() - q.submitted).count();
+    d.queue_ms = std::chrono::duration<double, std::milli>(b.launched - q.submitted).count();
+    d.completed_s = std::chrono::duration<double>(now - S->t0).count();
+    S->done.push_back(d);
+  }
+  S->requests += b.reqs.size();
+  b.reqs.clear();
+  b.busy = false;
+}
+
+void worker_main(mlod_server* S) {
+  (void)hipSetDevice(S->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  while (true) {
+    bool progressed = false;
+    bool anyBusy = false;
+    for (BatchSlot& b : S->slots) {
+      if (!b.busy) continue;
+      const hipError_t q = hipEventQuery(b.ev);
+      if (q == hipSuccess) {
+        complete_batch(S, b);
+        progressed = true;
+      } else if (q != hipErrorNotReady) {
+        S->errmsg = std::string("serving batch failed: ") + hipGetErrorString(q);
+        S->error = MLOD_ERR_HIP;
+      } else {
+        anyBusy = true;
+      }
+    }
+    if (progressed) S->cvDone.notify_all();
+    if (S->error) {
+      S->cvDone.notify_all();
+      return;
+    }
+    if (!S->pending.empty()) {
+      for (BatchSlot& b : S->slots) {
+        if (b.busy || S->pending.empty()) continue;
+        const int r = launch_batch(S, b);
+        if (r) {
+          S->errmsg = g_last_error();
+          S->error = r;
+          S->cvDone.notify_all();
+          return;
+        }
+        anyBusy = anyBusy || b.busy;
+        progressed = true;
+      }
+    }
+    if (S->stop && S->pending.empty() && !anyBusy) return;
+    if (!progressed) {
+      if (anyBusy) {
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        lk.lock();
+      } else {
+        S->cv.wait(lk, [S] { return S->stop || !S->pending.empty(); });
+      }
+    }
+  }
+}
+
+}  // namespace
+}  // namespace mlod
+
+using namespace mlod;
+
+extern "C" {
+
+int mlod_server_create(uint32_t max_batch_queries, uint32_t inflight, int32_t semantics, int32_t precision,
+                       mlod_server** out) {
+  if (!out) return fail(MLOD_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (max_batch_queries == 0 || inflight == 0 || inflight > 8) return fail(MLOD_ERR_INVALID, "bad batch capacity or inflight");
+  if (precision != MLOD_PREC_FP32 && precision != MLOD_PREC_FP32C)
+    return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
+  if (semantics != MLOD_SEM_REF_GPU && semantics != MLOD_SEM_REF_CPU) return fail(MLOD_ERR_INVALID, "bad semantics");
+  mlod_server* S = new mlod_server();
+  S->cap = max_batch_queries;
+  S->semantics = semantics;
+  S->precision = precision;
+  (void)hipGetDevice(&S->device);
+  S->slots.resize(inflight);
+  for (BatchSlot& b : S->slots) {
+    const hipError_t e = alloc_slot(b, S->cap);
+    if (e) {
+      for (BatchSlot& c : S->slots) free_slot(c);
+      delete S;
+      return hip_fail(e, "server buffers", __FILE__, __LINE__);
+    }
+  }
+  S->worker = std::thread(worker_main, S);
+  *out = S;
+  return MLOD_OK;
+}
+
+int mlod_server_destroy(mlod_server* S) {
+  if (!S) return MLOD_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  for (BatchSlot& b : S->slots) free_slot(b);
+  delete S;
+  return MLOD_OK;
+}
+
+int mlod_server_add_tree(mlod_server* S, mlod_tree* t, uint32_t* tree_id) {
+  if (!S || !t || !tree_id) return fail(MLOD_ERR_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  *tree_id = (uint32_t)S->trees.size();
+  S->trees.push_back(t);
+  return MLOD_OK;
+}
+
+int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, float cutoff, uint32_t min_depth,
+                       uint32_t n_iter, float step_max, const float* queries, uint32_t m, float* out_pts, float* out_nmls,
+                       uint64_t* ticket) {
+  if (!S || !k || !queries || !ticket) return fail(MLOD_ERR_INVALID, "NULL argument");
+  if (S->error) return fail(S->error, S->errmsg);
+  if (m == 0 || m > S->cap) return fail(MLOD_ERR_INVALID, "request size must be in 1..max_batch_queries");
+  if (k->type < 0 || k->type > 2) return fail(MLOD_ERR_INVALID, "bad kernel type");
+  if (k->type == MLOD_KERNEL_GAUSSIAN_MULTIPLE && (k->n_sigmas < 1 || k->n_sigmas > kServeMaxSigmas))
+    return fail(MLOD_ERR_UNSUPPORTED, "serving takes 1..8 sigmas");
+  SReq r;
+  r.tree = tree_id;
+  r.kernel = *k;
+  r.cutoff = cutoff;
+  r.minDepth = min_depth;
+  r.nIter = n_iter;
+  r.stepMax = step_max;
+  r.in = queries;
+  r.outP = out_pts;
+  r.outN = out_nmls;
+  r.m = m;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (tree_id >= S->trees.size()) return fail(MLOD_ERR_INVALID, "unknown tree id");
+    r.ticket = S->nextTicket++;
+    r.submitted = clk::now();
+    S->pending.push_back(r);
+  }
+  S->cv.notify_one();
+  *ticket = r.ticket;
+  return MLOD_OK;
+}
+
+int mlod_server_poll(mlod_server* S, mlod_server_done* out, uint32_t cap, uint32_t* n_out, double timeout_s) {
+  if (!S || !n_out) return fail(MLOD_ERR_INVALID, "NULL argument");
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (S->done.empty() && timeout_s > 0)
+    S->cvDone.wait_for(lk, std::chrono::duration<double>(timeout_s), [S] { return !S->done.empty() || S->error; });
+  if (S->error && S->done.empty()) return fail(S->error, S->errmsg);
+  const uint32_t n = (uint32_t)std::min<size_t>(cap, S->done.size());
+  for (uint32_t i = 0; i < n; ++i) out[i] = S->done[i];
+  S->done.erase(S->done.begin(), S->done.begin() + n);
+  *n_out = n;
+  return MLOD_OK;
+}
+
+int mlod_server_stats(mlod_server* S, uint64_t* out4) {
+  if (!S || !out4) return fail(MLOD_ERR_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  out4[0] = S->batches;
+  out4[1] = S->requests;
+  out4[2] = S->queries;
+  out4[3] = S->pending.size();
+  return MLOD_OK;
+}
+
+}  // extern "C"
