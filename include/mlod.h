@@ -164,6 +164,35 @@ int mlod_server_poll(mlod_server* s, mlod_server_done* out, uint32_t cap, uint32
 /* batches launched, requests completed, queries launched, requests pending */
 int mlod_server_stats(mlod_server* s, uint64_t* out4);
 
+/* ---------------------------------------------------------------------------------------
+ * Adaptive octree: OctreeGenerator (sources/scene/upsample.h:26-52).  Cells are octree ids
+ * (depth, x, y, z) (octree_id, sources/dual-contouring/octree.h:18-44) returned as int32 rows in
+ * the reference's node_comparator order (depth, then z, y, x; octree.h:147-154).  generate works
+ * in the [-1, 1]^3 frame of the reference (upsample.cpp:319-322) and projects with the tree's
+ * current kernel, parameters and neighbourhood mode; it overwrites the tree's uploaded queries
+ * (the reference re-uploads them too, upsample.cpp:331-332). */
+typedef struct mlod_generator mlod_generator;
+/* OctreeGenerator(APSS*, const Kernel&) (upsample.h:28); t may be NULL for generate_from_points. */
+int mlod_generator_create(mlod_tree* t, mlod_generator** out);
+int mlod_generator_destroy(mlod_generator* g);
+/* OctreeGenerator::generate (upsample.cpp:228-251), max_depth <= 12.  keep_total != 0 also
+ * collects total_tree, the cells crossed (scene.cpp:162-164). */
+int mlod_generate(mlod_generator* g, uint32_t max_depth, int32_t keep_total, void* stream);
+/* OctreeGenerator::generateFromPoints (upsample.cpp:128-144): the max_depth cells enclosing n
+ * points (host array, or device when device_src != 0) of box6 = (min xyz, max xyz); NULL or an
+ * empty box means [-1, 1]^3. */
+int mlod_generate_from_points(mlod_generator* g, const float* xyz, uint32_t n, int32_t device_src, uint32_t max_depth,
+                              const float* box6, void* stream);
+/* [leaves, total cells, max relative depth, centres projected, projection calls,
+ *  projections with a unit normal but a non-finite position (mapped to the root)] */
+int mlod_generator_info(const mlod_generator* g, uint64_t* out6);
+/* which = 0: the generated cells (generate's result); 1: total_tree.  out_ids4: n x 4 int32. */
+int mlod_generator_cells(mlod_generator* g, int32_t which, int32_t* out_ids4, void* stream);
+/* Centres of the generated cells (aabb_octree::center, octree.h:952-955) in the generator's
+ * frame: copied to out_xyz (host, n x 3) when non-NULL; *d_xyz receives the device array, valid
+ * until the next call on g.  These are the query points of the reference's upsampling. */
+int mlod_generator_centres(mlod_generator* g, float* out_xyz, const float** d_xyz, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

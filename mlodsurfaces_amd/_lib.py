@@ -19,6 +19,7 @@ KERNEL_GAUSSIAN, KERNEL_SINGULAR, KERNEL_GAUSSIAN_MULTIPLE = 0, 1, 2
 SEM_REF_GPU, SEM_REF_CPU = 0, 1
 PREC_FP32, PREC_FP32C, PREC_FP64 = 0, 1, 2
 MODE_MLOD, MODE_GLOBAL, MODE_KNN, MODE_BALL = 0, 1, 2, 3
+ERR_INVALID, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_STATE = -1, -2, -3, -4, -5
 
 # every function the header declares: name -> (restype, argtypes)
 _vp = C.c_void_p
@@ -71,6 +72,13 @@ SIGNATURES = {
                                      _vp, C.c_uint32, _vp, _vp, _u64]),
     "mlod_server_poll": (C.c_int, [_vp, C.POINTER(ServerDoneC), C.c_uint32, _u, C.c_double]),
     "mlod_server_stats": (C.c_int, [_vp, _u64]),
+    "mlod_generator_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "mlod_generator_destroy": (C.c_int, [_vp]),
+    "mlod_generate": (C.c_int, [_vp, C.c_uint32, C.c_int32, _vp]),
+    "mlod_generate_from_points": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int32, C.c_uint32, _vp, _vp]),
+    "mlod_generator_info": (C.c_int, [_vp, _u64]),
+    "mlod_generator_cells": (C.c_int, [_vp, C.c_int32, _vp, _vp]),
+    "mlod_generator_centres": (C.c_int, [_vp, _vp, C.POINTER(_vp), _vp]),
 }
 
 _lib = None

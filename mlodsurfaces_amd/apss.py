@@ -352,3 +352,76 @@ class GlobalAPSS(APSS):
     """Brute-force APSS over all points (GlobalAPSS, GlobalAPSS.h:19-43)."""
 
     MODE = _lib.MODE_GLOBAL
+
+
+class OctreeGenerator:
+    """The adaptive-octree driver (OctreeGenerator, upsample.h:26-52) on the GPU.
+
+    Cells are returned as (n, 4) int32 rows (depth, x, y, z) of the [-1, 1]^3 octree in the
+    reference's node_comparator order.  ``generate`` projects with ``apss``'s tree and settings
+    (its installed kernel, cutoff, min depth, semantics, form and neighbourhood mode); the
+    ``kernel`` argument is kept for the reference's signature, which passes it to a project call
+    that ignores it (Fastapss.cu:439-455)."""
+
+    def __init__(self, apss: APSS | None = None, kernel: Kernel | None = None):
+        self.apss = apss
+        self.kernel = kernel
+        h = C.c_void_p()
+        check(lib().mlod_generator_create(apss.m_apssOctree.handle if apss is not None else None, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mlod_generator_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def generate(self, max_depth, total_nodes: list | None = None, stream=None):
+        """OctreeGenerator::generate (upsample.cpp:228-251).  When ``total_nodes`` is a list it
+        receives the crossed cells (total_tree) as one (m, 4) array."""
+        if self.apss is None:
+            raise MlodError(_lib.ERR_STATE, "generate needs an APSS")
+        self.apss._sync_params()
+        check(lib().mlod_generate(self._h, int(max_depth), int(total_nodes is not None), stream))
+        if total_nodes is not None:
+            total_nodes.append(self.cells(1, stream))
+        return self.cells(0, stream)
+
+    def generateFromPoints(self, max_depth, pts, box=None, stream=None):
+        """OctreeGenerator::generateFromPoints (upsample.cpp:128-144); box = (min xyz, max xyz)."""
+        p = _f32(pts, "points")
+        b = None if box is None else np.ascontiguousarray(np.asarray(box, np.float32).reshape(6))
+        check(lib().mlod_generate_from_points(self._h, _fp(p), len(p), 0, int(max_depth),
+                                              _fp(b) if b is not None else None, stream))
+        return self.cells(0, stream)
+
+    def info(self):
+        o = (C.c_uint64 * 6)()
+        check(lib().mlod_generator_info(self._h, o))
+        return dict(leaves=o[0], total=o[1], max_rel_depth=o[2], projected=o[3], calls=o[4], nonfinite=o[5])
+
+    def cells(self, which=0, stream=None):
+        n = self.info()["total" if which else "leaves"]
+        out = np.zeros((n, 4), np.int32)
+        if n:
+            check(lib().mlod_generator_cells(self._h, int(which), out.ctypes.data_as(C.c_void_p), stream))
+        return out
+
+    def centres(self, stream=None):
+        """Centres of the generated cells (aabb_octree::center), (n, 3) float32."""
+        n = self.info()["leaves"]
+        out = np.zeros((n, 3), np.float32)
+        d = C.c_void_p()
+        check(lib().mlod_generator_centres(self._h, _fp(out), C.byref(d), stream))
+        return out
+
+    def centres_device(self, stream=None):
+        """Device address of the centres (n x 3 float32), valid until the next call."""
+        d = C.c_void_p()
+        check(lib().mlod_generator_centres(self._h, None, C.byref(d), stream))
+        return d.value

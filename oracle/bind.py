@@ -65,6 +65,9 @@ _u64 = C.POINTER(C.c_uint64)
 _orc = None
 _ref = None
 
+# void (*)(const float* xyz, uint32_t n, float* out_pts, float* out_nmls, void* user)
+PROJ_CB = C.CFUNCTYPE(None, _f, C.c_uint32, _f, _f, _vp)
+
 
 def oracle_lib():
     global _orc
@@ -90,6 +93,15 @@ def oracle_lib():
         L.orc_project.argtypes = [_vp, C.POINTER(KernelC), C.POINTER(ParamsC), C.c_int32, _f, C.c_uint32, C.c_uint32,
                                   C.c_float, _d, _d, C.c_int32]
         L.orc_fit.argtypes = [_d, _d, C.c_double, C.c_int32, _d, _d]
+        L.orc_gen_create.restype = _vp
+        L.orc_gen_create.argtypes = [PROJ_CB, _vp]
+        L.orc_gen_free.argtypes = [_vp]
+        L.orc_gen_run.argtypes = [_vp, C.c_uint32]
+        L.orc_gen_from_points.argtypes = [_vp, _f, C.c_uint32, C.c_uint32, _f]
+        L.orc_gen_info.argtypes = [_vp, _u64]
+        L.orc_gen_ids.argtypes = [_vp, C.c_int32, _i]
+        L.orc_cell_center.argtypes = [_i, _f, _f]
+        L.orc_cell_ancestor.argtypes = [_i, C.c_int32, _i]
         _orc = L
     return _orc
 
@@ -270,3 +282,74 @@ class RefTree:
         out = np.zeros((len(q), k, 3), np.float32)
         self.L.ref_knn(self.h, _ptr(q, C.c_float), len(q), k, _ptr(out, C.c_float))
         return out
+
+
+class OracleGenerator:
+    """The oracle's restated OctreeGenerator (upsample.cpp:128-144, 228-251).
+
+    ``project(centres float32 (n, 3)) -> (points (n, 3), normals (n, 3))`` is the projection
+    step of find_stable_nodesGPU (one step, unbounded step length)."""
+
+    def __init__(self, project=None):
+        self.L = oracle_lib()
+        self._project = project
+        self.calls = []
+
+        def cb(xyz, n, out_p, out_n, user):
+            c = np.ctypeslib.as_array(xyz, shape=(n, 3)).copy()
+            p, nm = self._project(c)
+            np.ctypeslib.as_array(out_p, shape=(n, 3))[:] = np.asarray(p, np.float32)
+            np.ctypeslib.as_array(out_n, shape=(n, 3))[:] = np.asarray(nm, np.float32)
+            self.calls.append(n)
+
+        self._cb = PROJ_CB(cb)
+        self.h = self.L.orc_gen_create(self._cb, None)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_gen_free(self.h)
+            self.h = None
+
+    def generate(self, max_depth):
+        self.L.orc_gen_run(self.h, int(max_depth))
+        return self.leaves()
+
+    def generate_from_points(self, points, max_depth, box6=None):
+        p = _c32(points)
+        b = None if box6 is None else _ptr(np.ascontiguousarray(box6, np.float32), C.c_float)
+        self.L.orc_gen_from_points(self.h, _ptr(p, C.c_float), len(p), int(max_depth), b)
+        return self.leaves()
+
+    def info(self):
+        o = np.zeros(6, np.uint64)
+        self.L.orc_gen_info(self.h, _ptr(o, C.c_uint64))
+        return dict(leaves=int(o[0]), total=int(o[1]), max_rel_depth=int(o[2]), projected=int(o[3]), calls=int(o[4]),
+                    bad=int(o[5]))
+
+    def _ids(self, which, n):
+        out = np.zeros((n, 4), np.int32)
+        self.L.orc_gen_ids(self.h, which, _ptr(out, C.c_int32))
+        return out
+
+    def leaves(self):
+        return self._ids(0, self.info()["leaves"])
+
+    def total(self):
+        return self._ids(1, self.info()["total"])
+
+
+def oracle_cell_center(id4, box6=None):
+    L = oracle_lib()
+    i = np.ascontiguousarray(id4, np.int32)
+    out = np.zeros(3, np.float32)
+    b = None if box6 is None else _ptr(np.ascontiguousarray(box6, np.float32), C.c_float)
+    L.orc_cell_center(_ptr(i, C.c_int32), b, _ptr(out, C.c_float))
+    return out
+
+
+def oracle_cell_ancestor(id4, depth):
+    L = oracle_lib()
+    i = np.ascontiguousarray(id4, np.int32)
+    out = np.zeros(4, np.int32)
+    L.orc_cell_ancestor(_ptr(i, C.c_int32), int(depth), _ptr(out, C.c_int32))
+    return out
