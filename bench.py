@@ -89,6 +89,47 @@ def cpu_baseline(w, kern, cutoff, budget_s=15.0):
                       f"with OpenMP on {cores} host threads; reference octree build {build_s:.1f} s untimed"}
 
 
+def traffic_from_profile(kernel_name, workload):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by the factor a plain copy
+    kernel of known size reads for the same counter), or None if no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_name}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1].split()[0])
+    return None
+
+
+def serve_measure(n_clouds=64, seed_rate=2000.0):
+    """Serving (64 resident clouds, Poisson requests of 1k-500k queries, mixed kernels, 10
+    steps each): capacity with all requests queued at once, then p50/p99 latency at the offered
+    2000 req/s and at a rate the server sustains (70% of its measured capacity)."""
+    from mlodsurfaces_amd import serve as SV
+    from mlodsurfaces_amd.apss import APSSOctree
+    t0 = time.time()
+    clouds = SV.serving_workload(n_clouds)
+    trees = [APSSOctree().build(p, n) for p, n in clouds]
+    npts = sum(len(c[0]) for c in clouds)
+    log(f"serve: {len(clouds)} clouds, {npts / 1e6:.1f}M points resident, ready in {time.time() - t0:.1f}s")
+    srv = SV.Server(max_batch=524288, inflight=3)
+    # capacity: a Poisson request mix submitted at once (offered rate far above service)
+    cap, _, _ = SV.run_load(srv, trees, clouds, rate=1e6, seconds=1.0, max_requests=400, seed=11, log=log)
+    mean_pts = cap["projected_pts"] / max(cap["requests"], 1)
+    cap_rate = cap["pts_per_s"] / mean_pts
+    spec, _, _ = SV.run_load(srv, trees, clouds, rate=seed_rate, seconds=0.5, seed=12, log=log)
+    sus_rate = 0.7 * cap_rate
+    sus, _, _ = SV.run_load(srv, trees, clouds, rate=sus_rate, seconds=3.0, seed=13, log=log)
+    srv.close()
+    keys = ("requests", "offered_rate", "pts_per_s", "p50_ms", "p99_ms", "p50_100k_ms", "p99_100k_ms", "n_100k", "wall_s")
+    return {"clouds": len(clouds), "resident_points": npts, "max_batch": 524288, "inflight": 3,
+            "capacity_pts_per_s": cap["pts_per_s"], "capacity_req_per_s": cap_rate,
+            "at_2000_req_s": {k: spec[k] for k in keys},
+            "at_70pct_capacity": {k: sus[k] for k in keys},
+            "note": "latency = submit -> results in the caller's host buffers; 100k = requests of 80k-125k queries"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +142,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the plain FP32 / FP64 comparison runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-clouds", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only.  The projection path would shard by "
@@ -191,6 +234,18 @@ def main():
                 "compulsory_bytes_per_launch": comp_bytes,
                 "compulsory_hbm_frac": comp_bytes / mean_launch_s / 1e9 / PEAK_HBM_GBS,
                 "mean_launch_ms": mean_launch_s * 1e3}
+    if args.workload == "superquadric8m" and args.scale == 1.0 and args.precision == "fp32c" and \
+            args.semantics == "ref_gpu":
+        roofline["traffic"] = traffic_from_profile("k_project", args.workload)
+    del tree, dP, dN, dQ
+    hiprt.device_synchronize()
+    serving = None
+    if not args.no_serve:
+        try:
+            serving = serve_measure(args.serve_clouds)
+        except Exception as ex:
+            log(f"serving measurement failed: {ex}")
+            raise
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -222,6 +277,7 @@ def main():
         "counters_per_step": counts,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -206,6 +206,7 @@ class APSS {
   const APSSOctree* CPU() const { return m_apssOctree; }
 
  protected:
+  friend class OctreeGenerator;
   APSS(APSSOctree* octree, int mode) : m_apssOctree(octree), m_mode(mode) {
     m_scalingProtectionSphere = octree->getScalingProtectionSphere();
     m_minDepth = octree->getMinDepth();
@@ -244,6 +245,65 @@ class GlobalAPSS : public APSS {
   GlobalAPSS(APSSOctree* apssOctree, const Kernel& kernel, unsigned nbOfVectors) : GlobalAPSS(apssOctree) {
     copyToGPU(kernel, nbOfVectors);
   }
+};
+
+// octree_id (sources/dual-contouring/octree.h:18-44): a cell of the [-1, 1]^3 octree.
+struct octree_id {
+  int depth;
+  int pos[3];
+};
+
+// OctreeGenerator (sources/scene/upsample.h:26-52), run on the GPU.  generate projects with
+// apss's tree, kernel and settings, as find_stable_nodesGPU does (upsample.cpp:317-346); the
+// cells come back in the reference's node_comparator order.
+class OctreeGenerator {
+ public:
+  OctreeGenerator(APSS* apss, const Kernel& kernel) : apss(apss), kernel(kernel) {
+    check(mlod_generator_create(apss ? apss->m_apssOctree->handle() : nullptr, &m_gen));
+  }
+  OctreeGenerator(const OctreeGenerator&) = delete;
+  OctreeGenerator& operator=(const OctreeGenerator&) = delete;
+  ~OctreeGenerator() { mlod_generator_destroy(m_gen); }
+
+  // OctreeGenerator::generate (upsample.cpp:228-251)
+  std::vector<octree_id> generate(unsigned int max_depth, std::vector<octree_id>* total_nodes = nullptr) const {
+    if (!apss) throw Error(MLOD_ERR_STATE, "generate needs an APSS");
+    apss->sync();
+    check(mlod_generate(m_gen, max_depth, total_nodes != nullptr, apss->m_stream));
+    if (total_nodes) *total_nodes = cells(1);
+    return cells(0);
+  }
+
+  // OctreeGenerator::generateFromPoints (upsample.cpp:128-144); box6 = (min xyz, max xyz),
+  // NULL for [-1, 1]^3.
+  template <class V>
+  std::vector<octree_id> generateFromPoints(unsigned int max_depth, const std::vector<V>& pts,
+                                            const float* box6 = nullptr) const {
+    check(mlod_generate_from_points(m_gen, fptr(pts.data()), (uint32_t)pts.size(), 0, max_depth, box6, nullptr));
+    return cells(0);
+  }
+
+  // Centres of the generated cells (aabb_octree::center): the upsampling query points.
+  std::vector<float> centres() const {
+    uint64_t inf[6];
+    check(mlod_generator_info(m_gen, inf));
+    std::vector<float> out(3 * inf[0]);
+    check(mlod_generator_centres(m_gen, out.data(), nullptr, nullptr));
+    return out;
+  }
+
+ private:
+  std::vector<octree_id> cells(int which) const {
+    uint64_t inf[6];
+    check(mlod_generator_info(m_gen, inf));
+    std::vector<octree_id> out(which ? inf[1] : inf[0]);
+    static_assert(sizeof(octree_id) == 4 * sizeof(int32_t), "octree_id is four int32");
+    if (!out.empty()) check(mlod_generator_cells(m_gen, which, reinterpret_cast<int32_t*>(out.data()), nullptr));
+    return out;
+  }
+  APSS* apss;
+  const Kernel& kernel;
+  mlod_generator* m_gen = nullptr;
 };
 
 }  // namespace mlod

@@ -85,8 +85,9 @@ struct ReqDesc {
   float stepMax;
   uint32_t minDepth;
   uint32_t nIter;
-  uint32_t slot0;   // first slot of the request in the batch
+  uint32_t slot0;   // first slot of the request in the batch (requests packed back to back)
   uint32_t count;   // queries
+  uint32_t pad0;    // first slot of the request in the workgroup-aligned layout
   float org[3];     // tree box origin and Morton scale
   float scale[3];
 };
@@ -152,6 +153,19 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (e_ != hipSuccess) return ::mlod::hip_fail(e_, #expr, __FILE__, __LINE__); \
   } while (0)
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip, each with its own L2.
+// Queries are Morton-sorted, so consecutive workgroups touch the same octree nodes; this maps
+// the hardware workgroup index to a logical one such that each XCD walks one contiguous share
+// of the query range (a bijection on [0, gridDim.x)), keeping each L2's working set disjoint.
+#if defined(__HIPCC__)
+__device__ __forceinline__ uint32_t xcd_block() {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t b = blockIdx.x, nb = gridDim.x;
+  const uint32_t x = b % kXcds, idx = b / kXcds, q = nb / kXcds, r = nb % kXcds;
+  return x * q + (x < r ? x : r) + idx;
+}
+#endif
+
 // build.hip
 int build_tree(mlod_tree* t, const float* d_xyz, const float* d_nrm, hipStream_t s);
 int dump_tree(const mlod_tree* t, int32_t* depth, float* bb6, float* center3, float* radius, int32_t* father,
@@ -163,10 +177,11 @@ KParams make_kparams(const mlod_kernel& k);
 int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s);
 int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream_t s);
 
-// serving batch (project.hip): requests [r0, r1) of `reqs` with kernel type kt, slots sorted so
-// that slot i belongs to request slotReq[i]
-int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_slotReq, float4* d_q, float4* d_qn, uint32_t begin,
-                         uint32_t end, int kt, int semantics, int precision, hipStream_t s);
+// serving batch (project.hip): the workgroup-aligned slots of workgroups [blk0, blk1) (each
+// workgroup of kBatchWG slots belongs to request blockReq[block]) with kernel type kt
+constexpr uint32_t kBatchWG = 128;
+int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_blockReq, float4* d_q, float4* d_qn, uint32_t blk0,
+                         uint32_t blk1, int kt, int semantics, int precision, hipStream_t s);
 WCoef make_wcoef(const mlod_kernel& k);
 
 // neighbors.hip

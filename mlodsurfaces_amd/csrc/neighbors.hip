@@ -264,7 +264,7 @@ __global__ __launch_bounds__(NBS) void k_knn(TreeView tv, const float* __restric
   __shared__ uint32_t sFirst[kLevels][NBS];
   __shared__ uint8_t sMask[kLevels][NBS];
   const int tid = threadIdx.x;
-  const uint32_t i = blockIdx.x * NBS + tid;
+  const uint32_t i = xcd_block() * NBS + tid;
   if (i >= m) return;
   if (limit < m && perm[i] >= limit) return;
   const float4 q4 = q[i];
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(NBS) void k_ball(TreeView tv, const float* __restri
   __shared__ uint32_t sFirst[kLevels][NBS];
   __shared__ uint8_t sMask[kLevels][NBS];
   const int tid = threadIdx.x;
-  const uint32_t i = blockIdx.x * NBS + tid;
+  const uint32_t i = xcd_block() * NBS + tid;
   if (i >= m) return;
   if (limit < m && perm[i] >= limit) return;
   const float4 q4 = q[i];

@@ -186,12 +186,31 @@ __device__ __forceinline__ void add_point(typename FormT<FORM>::Acc& acc, const 
   }
 }
 
-// Contribution of an internal node's moments with multiplier f, weight at its mean point.
-template <int FORM, int KT, class KP>
-__device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv, uint32_t c,
-                                         float Af, V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f) {
+// Moments of one node as loaded from HBM (node-centred FP32 for the compensated form,
+// global-frame FP32 for the plain form, FP64 for the FP64 form).
+template <int FORM> struct NodeMom { float4 m0, m1; };
+template <> struct NodeMom<2> { double4 m0, m1; double A; };
+
+template <int FORM>
+__device__ __forceinline__ NodeMom<FORM> load_mom(const TreeView& tv, uint32_t c) {
+  NodeMom<FORM> r;
   if constexpr (FORM == 0) {
-    const float4 m0 = tv.momP[2 * (size_t)c], m1 = tv.momP[2 * (size_t)c + 1];
+    r.m0 = tv.momP[2 * (size_t)c], r.m1 = tv.momP[2 * (size_t)c + 1];
+  } else if constexpr (FORM == 1) {
+    r.m0 = tv.momC[2 * (size_t)c], r.m1 = tv.momC[2 * (size_t)c + 1];
+  } else {
+    r.m0 = tv.momD[2 * (size_t)c], r.m1 = tv.momD[2 * (size_t)c + 1];
+    r.A = tv.AD[c];
+  }
+  return r;
+}
+
+// Contribution of a node's moments with multiplier f, weight at its mean point.
+template <int FORM, int KT, class KP>
+__device__ __forceinline__ void add_mom(typename FormT<FORM>::Acc& acc, const KP& kp, const NodeMom<FORM>& mm, float Af,
+                                        V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f) {
+  if constexpr (FORM == 0) {
+    const float4 m0 = mm.m0, m1 = mm.m1;
     const float dx = m0.x / Af - q.x, dy = m0.y / Af - q.y, dz = m0.z / Af - q.z;
     const float w = f * weight32<KT>(kp, dx * dx + dy * dy + dz * dz);
     acc.w = fmaf(w, Af, acc.w);
@@ -204,7 +223,7 @@ __device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const K
     acc.pn = fmaf(w, m0.w, acc.pn);
     acc.pp = fmaf(w, m1.w, acc.pp);
   } else if constexpr (FORM == 1) {
-    const float4 m0 = tv.momC[2 * (size_t)c], m1 = tv.momC[2 * (size_t)c + 1];
+    const float4 m0 = mm.m0, m1 = mm.m1;
     const float dx = m0.x - q.x, dy = m0.y - q.y, dz = m0.z - q.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
     const float fw = f * weight32<KT>(kp, d2);
@@ -219,8 +238,8 @@ __device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const K
     acc.pn.add(fw * (m0.w + (dx * m1.x + dy * m1.y + dz * m1.z)));
     acc.pp.add(fw * (m1.w + Af * d2));
   } else {
-    const double4 m0 = tv.momD[2 * (size_t)c], m1 = tv.momD[2 * (size_t)c + 1];
-    const double A = tv.AD[c];
+    const double4 m0 = mm.m0, m1 = mm.m1;
+    const double A = mm.A;
     const double dx = m0.x / A - q.x, dy = m0.y / A - q.y, dz = m0.z / A - q.z;
     const double w = f * weight64<KT>(kp, dx * dx + dy * dy + dz * dz);
     acc.w += w * A;
@@ -233,6 +252,12 @@ __device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const K
     acc.pn += w * m0.w;
     acc.pp += w * m1.w;
   }
+}
+
+template <int FORM, int KT, class KP>
+__device__ __forceinline__ void add_node(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv, uint32_t c,
+                                         float Af, V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T f) {
+  add_mom<FORM, KT>(acc, kp, load_mom<FORM>(tv, c), Af, q, f);
 }
 
 template <int FORM, int SEM, int KT, bool CNT, class KP>
@@ -256,6 +281,26 @@ __device__ __forceinline__ T dist3(V3<T> q, float4 g) {
   return fsqrt(dx * dx + dy * dy + dz * dz);
 }
 
+// Traversal arithmetic per form.  The plain forms keep IEEE division and square root (the
+// reference's arithmetic); the compensated form, whose error budget is set by its FP64 fit and
+// compensated sums, takes the hardware reciprocal and square root (1 ulp).
+template <int FORM>
+__device__ __forceinline__ typename FormT<FORM>::T tdist(V3<typename FormT<FORM>::T> q, float4 g) {
+  if constexpr (FORM == 1) {
+    const float dx = q.x - g.x, dy = q.y - g.y, dz = q.z - g.z;
+    return __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+  } else {
+    return dist3(q, g);
+  }
+}
+template <int FORM, class T>
+__device__ __forceinline__ T tratio(T a, T b) {  // a / b
+  if constexpr (FORM == 1)
+    return a * __builtin_amdgcn_rcpf(b);
+  else
+    return a / b;
+}
+
 template <class T>
 struct LevelState {
   uint32_t (*first)[PBS];
@@ -269,9 +314,15 @@ struct LevelState {
 // is added and the next pending near child anywhere on the lane's path becomes the current
 // node.  Either way an iteration handles exactly one node X: it loads X, computes its distance
 // and its blend factor against its parent, then either classifies X as a child (far / leaf /
-// near) or makes it the new current node, and ends in one shared add_node.  Lanes of a
-// wavefront therefore advance by one node per iteration whatever their position in their own
+// near) or makes it the new current node, and ends in one shared node contribution.  Lanes of
+// a wavefront therefore advance by one node per iteration whatever their position in their own
 // tree walk, instead of waiting for the widest child loop of the wavefront at every level.
+//
+// Each iteration issues all its global loads at once (X's geometry and header, and the moments
+// of the node it may contribute: X itself when testing, the finished node when moving on), so
+// one memory round trip covers the iteration.  Levels with near children still to expand are
+// kept in a per-lane bit set; the per-level state (first child, remaining near children,
+// multiplier, parent distance) lives in LDS and is only written when a second near child waits.
 template <int FORM, int SEM, int KT, bool CNT, class KP>
 __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv,
                          V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff, uint32_t minDepth,
@@ -283,27 +334,28 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
     return;
   }
   const float4 g0 = tv.geo[0];
-  const T d0 = dist3(q, g0);
+  const T d0 = tdist<FORM>(q, g0);
   if (minDepth == 0 && d0 > (T)g0.w * cutoff) {
     add_node<FORM, KT>(acc, kp, tv, 0, __uint_as_float(h0.z), q, T(1));
     if constexpr (CNT) ct.far_ += 1;
     return;
   }
   // current node = root
-  uint32_t node = 0, first = h0.x, nc = hdr_count(h0.y), k = 0, mask = 0;
+  uint32_t node = 0, first = h0.x, nc = hdr_count(h0.y), k = 0, mask = 0, pend = 0;
   int lvl = 0;
   bool blendNode = minDepth == 0;
   T m = T(1), distP = blendNode ? fabs(d0 - (T)g0.w * cutoff) : T(-1), shareSum = T(0);
   float Ai = __uint_as_float(h0.z);
+  T shareScale = SEM == MLOD_SEM_REF_CPU ? tratio<FORM>(T(1), (T)nc) : tratio<FORM>(T(1), (T)Ai);
   bool active = true;
   while (active) {
     // --- pick X: the next child of the current node, or the next pending node to expand
     const bool testing = k < nc;
-    uint32_t x;
-    T mp, dp;
+    uint32_t x = 0;
+    T mp = T(0), dp = T(-1);
     bool ownContrib = false;
-    uint32_t ownNode = node;
-    float ownA = Ai;
+    const uint32_t ownNode = node;
+    const float ownA = Ai;
     T ownF = T(0);
     if (testing) {
       x = first + k;
@@ -313,50 +365,62 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
       ownContrib = blendNode && shareSum != T(0);
       ownF = m * shareSum;
       if constexpr (CNT) ct.nearN += blendNode ? 1u : 0u;
-      st.first[lvl][tid] = first;
-      st.mask[lvl][tid] = (uint8_t)mask;
-      st.mul[lvl][tid] = m;
-      st.dist[lvl][tid] = blendNode ? distP : T(-1);
-      int L = lvl;
-      uint32_t pm = mask;
-      while (pm == 0u && L > 0) {
-        --L;
-        pm = st.mask[L][tid];
-      }
-      if (pm == 0u) {
-        active = false;
-        x = 0;
-        mp = T(0);
-        dp = T(-1);
-      } else {
-        st.mask[L][tid] = (uint8_t)(pm & (pm - 1u));
+      if (mask) {
+        // the finished node's first near child is next; the rest wait at this level
+        const uint32_t rest = mask & (mask - 1u);
+        if (rest) {
+          st.first[lvl][tid] = first;
+          st.mask[lvl][tid] = (uint8_t)rest;
+          st.mul[lvl][tid] = m;
+          st.dist[lvl][tid] = blendNode ? distP : T(-1);
+          pend |= 1u << lvl;
+        }
+        x = first + (uint32_t)__builtin_ctz(mask);
+        mp = m;
+        dp = blendNode ? distP : T(-1);
+        lvl = lvl + 1;
+      } else if (pend) {
+        const int L = 31 - __builtin_clz(pend);
+        const uint32_t pm = st.mask[L][tid];
+        const uint32_t rest = pm & (pm - 1u);
+        st.mask[L][tid] = (uint8_t)rest;
+        if (!rest) pend &= ~(1u << L);
         x = st.first[L][tid] + (uint32_t)__builtin_ctz(pm);
         mp = st.mul[L][tid];
         dp = st.dist[L][tid];
         lvl = L + 1;
+      } else {
+        active = false;
       }
     }
-    // --- X's geometry and blend factor against its parent (CPUUtils.cpp:489-497)
+    // --- one round of loads: X's geometry and header, and the moments that may be added
+    const uint32_t tgt = testing ? x : ownNode;
+    const NodeMom<FORM> mm = load_mom<FORM>(tv, tgt);
+    float4 gx = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint4 hx = make_uint4(0u, 0u, 0u, 0u);
+    if (active) {
+      gx = tv.geo[x];
+      hx = tv.hdr[x];
+    }
+    // --- X's blend factor against its parent (CPUUtils.cpp:489-497)
     bool contrib = ownContrib;
-    uint32_t tgt = ownNode;
     float At = ownA;
     T f = ownF;
     if (active) {
-      const float4 gx = tv.geo[x];
-      const uint4 hx = tv.hdr[x];
-      const T dxq = dist3(q, gx);
+      const T dxq = tdist<FORM>(q, gx);
       const T rc = (T)gx.w * cutoff;
       T beta = T(1);
       if (dp >= T(0)) {
         const T dc = fmax(dxq - rc, T(0));
-        beta = blend(dp / (dp + dc));
+        beta = blend(tratio<FORM>(dp, dp + dc));
       }
       const T mx = mp * beta;
       const float Ax = __uint_as_float(hx.z);
       if (testing) {
         ++k;
         if (blendNode) {
-          const T share = SEM == MLOD_SEM_REF_CPU ? T(1) / (T)nc : (T)Ax / (T)Ai;
+          const T share = SEM == MLOD_SEM_REF_CPU ? shareScale
+                                                  : (FORM == 1 ? (T)Ax * shareScale : (T)Ax / (T)Ai);
           shareSum += (T(1) - beta) * share;
         }
         if constexpr (CNT) ct.tested += 1;
@@ -370,7 +434,6 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
           ct.leafPts += single ? 1u : 0u;
         }
         contrib = far || single;
-        tgt = x;
         At = Ax;
         f = mx;
       } else {
@@ -385,9 +448,10 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
         m = mx;
         shareSum = T(0);
         Ai = Ax;
+        shareScale = SEM == MLOD_SEM_REF_CPU ? tratio<FORM>(T(1), (T)nc) : tratio<FORM>(T(1), (T)Ax);
       }
     }
-    if (contrib) add_node<FORM, KT>(acc, kp, tv, tgt, At, q, f);
+    if (contrib) add_mom<FORM, KT>(acc, kp, mm, At, q, f);
   }
 }
 
@@ -504,7 +568,7 @@ __global__ __launch_bounds__(PBS) void k_project(TreeView tv, KParams kp, float4
   __shared__ T sMul[kLevels][PBS];
   __shared__ T sDist[kLevels][PBS];
   const int tid = threadIdx.x;
-  const uint32_t i = blockIdx.x * PBS + tid;
+  const uint32_t i = xcd_block() * PBS + tid;
   if (i >= m) return;
   if (limit < m && perm[i] >= limit) return;
   const LevelState<T> st{sFirst, sMask, sMul, sDist};
@@ -581,22 +645,25 @@ __global__ __launch_bounds__(PBS) void k_global(TreeView tv, KParams kp, float4*
   }
 }
 
-// Serving batch: lane i projects slot i of the batch with the tree, kernel and parameters of
-// its request (a per-lane read of the request table); per-lane work and results are those of
-// k_project, so a query gives the same bits alone or in any batch.
+// Serving batch: workgroup b of the aligned layout projects kBatchWG slots of request
+// blockReq[b] with that request's tree, kernel and parameters, read once per workgroup into
+// scalar registers; per-lane work and results are those of k_project, so a query gives the same
+// bits alone or in any batch.
 template <int FORM, int SEM, int KT>
-__global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict__ reqs, const uint32_t* __restrict__ slotReq,
-                                                       float4* __restrict__ q, float4* __restrict__ qn, uint32_t begin,
-                                                       uint32_t end) {
+__global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict__ reqs, const uint32_t* __restrict__ blockReq,
+                                                       float4* __restrict__ q, float4* __restrict__ qn, uint32_t blk0) {
+  static_assert(PBS == kBatchWG, "one workgroup per aligned request block");
   using T = typename FormT<FORM>::T;
   __shared__ uint32_t sFirst[kLevels][PBS];
   __shared__ uint8_t sMask[kLevels][PBS];
   __shared__ T sMul[kLevels][PBS];
   __shared__ T sDist[kLevels][PBS];
   const int tid = threadIdx.x;
-  const uint32_t i = begin + blockIdx.x * PBS + tid;
-  if (i >= end) return;
-  const ReqDesc& R = reqs[slotReq[i]];
+  const uint32_t blk = blk0 + xcd_block();
+  const uint32_t r = __builtin_amdgcn_readfirstlane(blockReq[blk]);
+  const ReqDesc& R = reqs[r];
+  const uint32_t i = blk * PBS + tid;
+  if (i >= R.pad0 + R.count) return;
   const TreeView tv = R.tv;
   const WCoef w = R.w;
   const float cutoff = R.cutoff, stepMax = R.stepMax;
@@ -619,13 +686,13 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
 }
 
 template <int FORM, int SEM>
-void launch_batch_kt(const ReqDesc* r, const uint32_t* sr, float4* q, float4* qn, uint32_t b, uint32_t e, int kt,
+void launch_batch_kt(const ReqDesc* r, const uint32_t* br, float4* q, float4* qn, uint32_t b0, uint32_t b1, int kt,
                      hipStream_t s) {
-  const uint32_t grid = (e - b + PBS - 1) / PBS;
+  const uint32_t grid = b1 - b0;
   switch (kt) {
-    case MLOD_KERNEL_GAUSSIAN: k_project_batch<FORM, SEM, MLOD_KERNEL_GAUSSIAN><<<grid, PBS, 0, s>>>(r, sr, q, qn, b, e); break;
-    case MLOD_KERNEL_SINGULAR: k_project_batch<FORM, SEM, MLOD_KERNEL_SINGULAR><<<grid, PBS, 0, s>>>(r, sr, q, qn, b, e); break;
-    default: k_project_batch<FORM, SEM, MLOD_KERNEL_GAUSSIAN_MULTIPLE><<<grid, PBS, 0, s>>>(r, sr, q, qn, b, e); break;
+    case MLOD_KERNEL_GAUSSIAN: k_project_batch<FORM, SEM, MLOD_KERNEL_GAUSSIAN><<<grid, PBS, 0, s>>>(r, br, q, qn, b0); break;
+    case MLOD_KERNEL_SINGULAR: k_project_batch<FORM, SEM, MLOD_KERNEL_SINGULAR><<<grid, PBS, 0, s>>>(r, br, q, qn, b0); break;
+    default: k_project_batch<FORM, SEM, MLOD_KERNEL_GAUSSIAN_MULTIPLE><<<grid, PBS, 0, s>>>(r, br, q, qn, b0); break;
   }
 }
 
@@ -706,16 +773,16 @@ WCoef make_wcoef(const mlod_kernel& k) {
   return w;
 }
 
-int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_slotReq, float4* d_q, float4* d_qn, uint32_t begin,
-                         uint32_t end, int kt, int semantics, int precision, hipStream_t s) {
-  if (end <= begin) return MLOD_OK;
+int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_blockReq, float4* d_q, float4* d_qn, uint32_t blk0,
+                         uint32_t blk1, int kt, int semantics, int precision, hipStream_t s) {
+  if (blk1 <= blk0) return MLOD_OK;
   const bool cpu = semantics == MLOD_SEM_REF_CPU;
   if (precision == MLOD_PREC_FP32C) {
-    if (cpu) launch_batch_kt<1, MLOD_SEM_REF_CPU>(d_reqs, d_slotReq, d_q, d_qn, begin, end, kt, s);
-    else launch_batch_kt<1, MLOD_SEM_REF_GPU>(d_reqs, d_slotReq, d_q, d_qn, begin, end, kt, s);
+    if (cpu) launch_batch_kt<1, MLOD_SEM_REF_CPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, kt, s);
+    else launch_batch_kt<1, MLOD_SEM_REF_GPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, kt, s);
   } else if (precision == MLOD_PREC_FP32) {
-    if (cpu) launch_batch_kt<0, MLOD_SEM_REF_CPU>(d_reqs, d_slotReq, d_q, d_qn, begin, end, kt, s);
-    else launch_batch_kt<0, MLOD_SEM_REF_GPU>(d_reqs, d_slotReq, d_q, d_qn, begin, end, kt, s);
+    if (cpu) launch_batch_kt<0, MLOD_SEM_REF_CPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, kt, s);
+    else launch_batch_kt<0, MLOD_SEM_REF_GPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, kt, s);
   } else {
     return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
   }

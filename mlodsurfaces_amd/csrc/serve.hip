@@ -10,11 +10,14 @@
 //
 // One batch on its stream: pinned host staging -> device copy; per-slot Morton key in the
 // request's tree box with the request index above it; radix sort (so each wavefront walks one
-// request's neighbourhood); gather; one projection launch per kernel type over that type's
-// contiguous slot range (k_project_batch: per-lane request table, the same per-query code and
-// bits as a standalone projection); scatter back to submission order; device -> pinned copy;
-// completion event.  The worker copies each request's results to the caller's buffers when
-// the event has fired and stamps the completion time.
+// request's neighbourhood); gather into a layout where every request starts on a workgroup
+// boundary; one projection launch per kernel type over that type's workgroups
+// (k_project_batch: the workgroup's request descriptor is uniform and read into scalar
+// registers; per-query code and bits are those of a standalone projection); scatter back to
+// submission order; device -> pinned copy; completion event.  The worker copies each request's
+// results to the caller's buffers when the event has fired and stamps the completion time.
+// Only the request queue and the completion list are shared with callers (under the mutex);
+// staging copies and launches run outside it.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -72,26 +75,32 @@ __global__ __launch_bounds__(BS) void k_batch_keys(const float* __restrict__ in,
   idx[i] = i;
 }
 
-__global__ __launch_bounds__(BS) void k_batch_gather(const float* __restrict__ in, const uint64_t* __restrict__ key,
-                                                     const uint32_t* __restrict__ perm, uint32_t slots,
-                                                     float4* __restrict__ q, float4* __restrict__ qn,
-                                                     uint32_t* __restrict__ slotReq) {
-  const uint32_t j = blockIdx.x * BS + threadIdx.x;
-  if (j >= slots) return;
-  const uint32_t s = perm[j];
-  q[j] = make_float4(in[3 * (uint64_t)s], in[3 * (uint64_t)s + 1], in[3 * (uint64_t)s + 2], 0.f);
-  qn[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-  slotReq[j] = (uint32_t)(key[j] >> 30);
+// sorted slot j (request key[j] >> 30) -> its place in the workgroup-aligned layout
+__device__ __forceinline__ uint32_t aligned_slot(const ReqDesc* __restrict__ reqs, uint64_t key, uint32_t j) {
+  const ReqDesc& R = reqs[(uint32_t)(key >> 30)];
+  return R.pad0 + (j - R.slot0);
 }
 
-// out = [slots x 3 positions | slots x 3 normals] in submission order
+__global__ __launch_bounds__(BS) void k_batch_gather(const float* __restrict__ in, const uint64_t* __restrict__ key,
+                                                     const uint32_t* __restrict__ perm, uint32_t slots,
+                                                     const ReqDesc* __restrict__ reqs, float4* __restrict__ q,
+                                                     float4* __restrict__ qn) {
+  const uint32_t j = blockIdx.x * BS + threadIdx.x;
+  if (j >= slots) return;
+  const uint32_t s = perm[j], d = aligned_slot(reqs, key[j], j);
+  q[d] = make_float4(in[3 * (uint64_t)s], in[3 * (uint64_t)s + 1], in[3 * (uint64_t)s + 2], 0.f);
+  qn[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// out = [cap x 3 positions | cap x 3 normals] in submission order
 __global__ __launch_bounds__(BS) void k_batch_scatter(const float4* __restrict__ q, const float4* __restrict__ qn,
-                                                      const uint32_t* __restrict__ perm, uint32_t slots, uint32_t cap,
+                                                      const uint64_t* __restrict__ key, const uint32_t* __restrict__ perm,
+                                                      const ReqDesc* __restrict__ reqs, uint32_t slots, uint32_t cap,
                                                       float* __restrict__ out) {
   const uint32_t j = blockIdx.x * BS + threadIdx.x;
   if (j >= slots) return;
-  const uint32_t d = perm[j];
-  const float4 p = q[j], n = qn[j];
+  const uint32_t d = perm[j], a = aligned_slot(reqs, key[j], j);
+  const float4 p = q[a], n = qn[a];
   out[3 * (uint64_t)d] = p.x;
   out[3 * (uint64_t)d + 1] = p.y;
   out[3 * (uint64_t)d + 2] = p.z;
@@ -129,7 +138,8 @@ struct BatchSlot {
   uint64_t* dKey2 = nullptr;
   uint32_t* dIdx = nullptr;
   uint32_t* dPerm = nullptr;
-  uint32_t* dSlotReq = nullptr;
+  uint32_t* dBlockReq = nullptr;   // request of each workgroup of the aligned layout
+  uint32_t* hBlockReq = nullptr;
   ReqDesc* dReq = nullptr;
   void* sortTmp = nullptr;
   size_t sortBytes = 0;
@@ -176,10 +186,10 @@ hipError_t hmalloc(T** p, size_t n) {
 
 void free_slot(BatchSlot& b) {
   if (b.s) (void)hipStreamSynchronize(b.s);
-  void* d[] = {b.dIn, b.dOut, b.dQ, b.dQn, b.dKey, b.dKey2, b.dIdx, b.dPerm, b.dSlotReq, b.dReq, b.sortTmp};
+  void* d[] = {b.dIn, b.dOut, b.dQ, b.dQn, b.dKey, b.dKey2, b.dIdx, b.dPerm, b.dBlockReq, b.dReq, b.sortTmp};
   for (void* p : d)
     if (p) (void)hipFree(p);
-  void* h[] = {b.hIn, b.hOut, b.hReq};
+  void* h[] = {b.hIn, b.hOut, b.hReq, b.hBlockReq};
   for (void* p : h)
     if (p) (void)hipHostFree(p);
   if (b.ev) (void)hipEventDestroy(b.ev);
@@ -187,23 +197,27 @@ void free_slot(BatchSlot& b) {
   b = BatchSlot{};
 }
 
+// slots of the workgroup-aligned layout: every request may waste up to kBatchWG - 1
+inline size_t aligned_cap(uint32_t cap) { return (size_t)cap + (size_t)kMaxReqPerBatch * kBatchWG; }
+
 hipError_t alloc_slot(BatchSlot& b, uint32_t cap) {
   hipError_t e;
+  const size_t acap = aligned_cap(cap), nblk = acap / kBatchWG;
   if ((e = hipStreamCreate(&b.s)) || (e = hipEventCreateWithFlags(&b.ev, hipEventDisableTiming))) return e;
   if ((e = hmalloc(&b.hIn, 3 * (size_t)cap)) || (e = hmalloc(&b.hOut, 6 * (size_t)cap)) ||
-      (e = hmalloc(&b.hReq, kMaxReqPerBatch)))
+      (e = hmalloc(&b.hReq, kMaxReqPerBatch)) || (e = hmalloc(&b.hBlockReq, nblk)))
     return e;
-  if ((e = dmalloc(&b.dIn, 3 * (size_t)cap)) || (e = dmalloc(&b.dOut, 6 * (size_t)cap)) || (e = dmalloc(&b.dQ, cap)) ||
-      (e = dmalloc(&b.dQn, cap)) || (e = dmalloc(&b.dKey, cap)) || (e = dmalloc(&b.dKey2, cap)) ||
-      (e = dmalloc(&b.dIdx, cap)) || (e = dmalloc(&b.dPerm, cap)) || (e = dmalloc(&b.dSlotReq, cap)) ||
+  if ((e = dmalloc(&b.dIn, 3 * (size_t)cap)) || (e = dmalloc(&b.dOut, 6 * (size_t)cap)) || (e = dmalloc(&b.dQ, acap)) ||
+      (e = dmalloc(&b.dQn, acap)) || (e = dmalloc(&b.dKey, cap)) || (e = dmalloc(&b.dKey2, cap)) ||
+      (e = dmalloc(&b.dIdx, cap)) || (e = dmalloc(&b.dPerm, cap)) || (e = dmalloc(&b.dBlockReq, nblk)) ||
       (e = dmalloc(&b.dReq, kMaxReqPerBatch)))
     return e;
   if ((e = rocprim::radix_sort_pairs(nullptr, b.sortBytes, b.dKey, b.dKey2, b.dIdx, b.dPerm, cap, 0, 40, b.s))) return e;
   return hipMalloc(&b.sortTmp, b.sortBytes + 16);
 }
 
-// Packs pending requests (oldest first) into slot b and launches its work.  Caller holds mu.
-int launch_batch(mlod_server* S, BatchSlot& b) {
+// Takes pending requests, oldest first, that fit one batch.  Caller holds mu.
+void take_requests(mlod_server* S, BatchSlot& b) {
   b.reqs.clear();
   uint32_t slots = 0;
   while (!S->pending.empty() && b.reqs.size() < kMaxReqPerBatch && slots + S->pending.front().m <= S->cap) {
@@ -211,11 +225,14 @@ int launch_batch(mlod_server* S, BatchSlot& b) {
     b.reqs.push_back(S->pending.front());
     S->pending.pop_front();
   }
-  if (b.reqs.empty()) return MLOD_OK;
-  // group by kernel type: each type is one contiguous slot range and one launch
+}
+
+// Stages and launches the requests in b.reqs (worker thread, without the mutex).
+int launch_batch(mlod_server* S, BatchSlot& b) {
+  // group by kernel type: each type is one contiguous run of workgroups and one launch
   std::stable_sort(b.reqs.begin(), b.reqs.end(),
                    [](const SReq& x, const SReq& y) { return x.kernel.type < y.kernel.type; });
-  uint32_t off = 0, typeBeg[3] = {0, 0, 0}, typeEnd[3] = {0, 0, 0};
+  uint32_t off = 0, blk = 0, typeBeg[3] = {0, 0, 0}, typeEnd[3] = {0, 0, 0};
   bool seen[3] = {false, false, false};
   for (size_t r = 0; r < b.reqs.size(); ++r) {
     const SReq& q = b.reqs[r];
@@ -229,26 +246,32 @@ int launch_batch(mlod_server* S, BatchSlot& b) {
     d.nIter = q.nIter;
     d.slot0 = off;
     d.count = q.m;
+    d.pad0 = blk * kBatchWG;
     for (int a = 0; a < 3; ++a) {
       const float ext = t->BB[a] - t->bb[a];
       d.org[a] = t->bb[a];
       d.scale[a] = ext > 0.f ? 1024.f / ext : 0.f;
     }
     std::memcpy(b.hIn + 3 * (size_t)off, q.in, (size_t)q.m * 3 * sizeof(float));
+    const uint32_t nb = (q.m + kBatchWG - 1) / kBatchWG;
+    for (uint32_t i = 0; i < nb; ++i) b.hBlockReq[blk + i] = (uint32_t)r;
     const int kt = q.kernel.type;
     if (!seen[kt]) {
-      typeBeg[kt] = off;
+      typeBeg[kt] = blk;
       seen[kt] = true;
     }
     off += q.m;
-    typeEnd[kt] = off;
+    blk += nb;
+    typeEnd[kt] = blk;
   }
+  const uint32_t slots = off;
   b.slots = slots;
   const uint32_t nReq = (uint32_t)b.reqs.size();
   int reqBits = 1;
   while ((1u << reqBits) < nReq) ++reqBits;
   hipError_t e = hipMemcpyAsync(b.dIn, b.hIn, (size_t)slots * 3 * sizeof(float), hipMemcpyHostToDevice, b.s);
   if (!e) e = hipMemcpyAsync(b.dReq, b.hReq, nReq * sizeof(ReqDesc), hipMemcpyHostToDevice, b.s);
+  if (!e) e = hipMemcpyAsync(b.dBlockReq, b.hBlockReq, blk * sizeof(uint32_t), hipMemcpyHostToDevice, b.s);
   const uint32_t grid = (slots + BS - 1) / BS;
   if (!e) {
     k_batch_keys<<<grid, BS, 0, b.s>>>(b.dIn, slots, b.dReq, nReq, b.dKey, b.dIdx);
@@ -256,17 +279,17 @@ int launch_batch(mlod_server* S, BatchSlot& b) {
   }
   if (!e) e = rocprim::radix_sort_pairs(b.sortTmp, b.sortBytes, b.dKey, b.dKey2, b.dIdx, b.dPerm, slots, 0, 30 + reqBits, b.s);
   if (!e) {
-    k_batch_gather<<<grid, BS, 0, b.s>>>(b.dIn, b.dKey2, b.dPerm, slots, b.dQ, b.dQn, b.dSlotReq);
+    k_batch_gather<<<grid, BS, 0, b.s>>>(b.dIn, b.dKey2, b.dPerm, slots, b.dReq, b.dQ, b.dQn);
     e = hipGetLastError();
   }
   if (e) return hip_fail(e, "serving batch", __FILE__, __LINE__);
   for (int kt = 0; kt < 3; ++kt) {
     if (!seen[kt]) continue;
-    const int r = launch_project_batch(b.dReq, b.dSlotReq, b.dQ, b.dQn, typeBeg[kt], typeEnd[kt], kt, S->semantics,
+    const int r = launch_project_batch(b.dReq, b.dBlockReq, b.dQ, b.dQn, typeBeg[kt], typeEnd[kt], kt, S->semantics,
                                        S->precision, b.s);
     if (r) return r;
   }
-  k_batch_scatter<<<grid, BS, 0, b.s>>>(b.dQ, b.dQn, b.dPerm, slots, S->cap, b.dOut);
+  k_batch_scatter<<<grid, BS, 0, b.s>>>(b.dQ, b.dQn, b.dKey2, b.dPerm, b.dReq, slots, S->cap, b.dOut);
   e = hipGetLastError();
   if (!e) e = hipMemcpyAsync(b.hOut, b.dOut, (size_t)slots * 3 * sizeof(float), hipMemcpyDeviceToHost, b.s);
   if (!e)
@@ -274,80 +297,96 @@ int launch_batch(mlod_server* S, BatchSlot& b) {
                        hipMemcpyDeviceToHost, b.s);
   if (!e) e = hipEventRecord(b.ev, b.s);
   if (e) return hip_fail(e, "serving batch", __FILE__, __LINE__);
-  b.busy = true;
   b.launched = clk::now();
-  S->batches += 1;
-  S->queries += slots;
   return MLOD_OK;
 }
 
-// Copies results of a finished batch to the callers.  Caller holds mu.
-void complete_batch(mlod_server* S, BatchSlot& b) {
-  const auto now = clk::now();
+// Copies a finished batch's results to the callers (worker thread, without the mutex) and
+// returns their completion records.
+void complete_batch(mlod_server* S, BatchSlot& b, std::vector<mlod_server_done>& out) {
   for (size_t r = 0; r < b.reqs.size(); ++r) {
     const SReq& q = b.reqs[r];
     const uint32_t off = b.hReq[r].slot0;
     if (q.outP) std::memcpy(q.outP, b.hOut + 3 * (size_t)off, (size_t)q.m * 3 * sizeof(float));
     if (q.outN) std::memcpy(q.outN, b.hOut + 3 * ((size_t)S->cap + off), (size_t)q.m * 3 * sizeof(float));
+  }
+  const auto now = clk::now();
+  for (const SReq& q : b.reqs) {
     mlod_server_done d;
     d.ticket = q.ticket;
-    d.latency_ms = std::chrono::duration<double, std::milli>(clk::now() - q.submitted).count();
+    d.latency_ms = std::chrono::duration<double, std::milli>(now - q.submitted).count();
     d.queue_ms = std::chrono::duration<double, std::milli>(b.launched - q.submitted).count();
     d.completed_s = std::chrono::duration<double>(now - S->t0).count();
-    S->done.push_back(d);
+    out.push_back(d);
   }
-  S->requests += b.reqs.size();
-  b.reqs.clear();
-  b.busy = false;
+}
+
+void set_error(mlod_server* S, int code, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (!S->error) {
+    S->errmsg = msg;
+    S->error = code;
+  }
+  S->cvDone.notify_all();
 }
 
 void worker_main(mlod_server* S) {
   (void)hipSetDevice(S->device);
-  std::unique_lock<std::mutex> lk(S->mu);
+  std::vector<mlod_server_done> fin;
   while (true) {
-    bool progressed = false;
-    bool anyBusy = false;
+    bool progressed = false, anyBusy = false;
+    // completions: copy out, then publish
     for (BatchSlot& b : S->slots) {
       if (!b.busy) continue;
       const hipError_t q = hipEventQuery(b.ev);
       if (q == hipSuccess) {
-        complete_batch(S, b);
+        fin.clear();
+        complete_batch(S, b, fin);
+        const size_t nr = b.reqs.size();
+        b.reqs.clear();
+        b.busy = false;
+        {
+          std::lock_guard<std::mutex> lk(S->mu);
+          S->done.insert(S->done.end(), fin.begin(), fin.end());
+          S->requests += nr;
+        }
+        S->cvDone.notify_all();
         progressed = true;
       } else if (q != hipErrorNotReady) {
-        S->errmsg = std::string("serving batch failed: ") + hipGetErrorString(q);
-        S->error = MLOD_ERR_HIP;
+        set_error(S, MLOD_ERR_HIP, std::string("serving batch failed: ") + hipGetErrorString(q));
+        return;
       } else {
         anyBusy = true;
       }
     }
-    if (progressed) S->cvDone.notify_all();
-    if (S->error) {
-      S->cvDone.notify_all();
-      return;
-    }
-    if (!S->pending.empty()) {
-      for (BatchSlot& b : S->slots) {
-        if (b.busy || S->pending.empty()) continue;
-        const int r = launch_batch(S, b);
-        if (r) {
-          S->errmsg = g_last_error();
-          S->error = r;
-          S->cvDone.notify_all();
-          return;
-        }
-        anyBusy = anyBusy || b.busy;
-        progressed = true;
+    // launches into free slots
+    for (BatchSlot& b : S->slots) {
+      if (b.busy) continue;
+      {
+        std::lock_guard<std::mutex> lk(S->mu);
+        take_requests(S, b);
       }
+      if (b.reqs.empty()) break;
+      const int r = launch_batch(S, b);
+      if (r) {
+        set_error(S, r, g_last_error());
+        return;
+      }
+      b.busy = true;
+      anyBusy = true;
+      progressed = true;
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->batches += 1;
+      S->queries += b.slots;
     }
+    std::unique_lock<std::mutex> lk(S->mu);
     if (S->stop && S->pending.empty() && !anyBusy) return;
-    if (!progressed) {
-      if (anyBusy) {
-        lk.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(20));
-        lk.lock();
-      } else {
-        S->cv.wait(lk, [S] { return S->stop || !S->pending.empty(); });
-      }
+    if (progressed) continue;
+    if (anyBusy) {
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    } else {
+      S->cv.wait(lk, [S] { return S->stop || !S->pending.empty(); });
     }
   }
 }
@@ -411,7 +450,10 @@ int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, f
                        uint32_t n_iter, float step_max, const float* queries, uint32_t m, float* out_pts, float* out_nmls,
                        uint64_t* ticket) {
   if (!S || !k || !queries || !ticket) return fail(MLOD_ERR_INVALID, "NULL argument");
-  if (S->error) return fail(S->error, S->errmsg);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->error) return fail(S->error, S->errmsg);
+  }
   if (m == 0 || m > S->cap) return fail(MLOD_ERR_INVALID, "request size must be in 1..max_batch_queries");
   if (k->type < 0 || k->type > 2) return fail(MLOD_ERR_INVALID, "bad kernel type");
   if (k->type == MLOD_KERNEL_GAUSSIAN_MULTIPLE && (k->n_sigmas < 1 || k->n_sigmas > kServeMaxSigmas))

@@ -38,7 +38,7 @@ def test_dropin_matches_python_api(tmp_path):
     q = ds.jitter_queries(p[:3000], 0.01, seed=42)
     for name, a in (("p", p), ("n", n), ("q", q)):
         a.astype(np.float32).tofile(tmp_path / f"{name}.bin")
-    r = subprocess.run([exe, *(str(tmp_path / f) for f in ("p.bin", "n.bin", "q.bin", "op.bin", "on.bin"))],
+    r = subprocess.run([exe, *(str(tmp_path / f) for f in ("p.bin", "n.bin", "q.bin", "op.bin", "on.bin", "cells.bin"))],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     op = np.fromfile(tmp_path / "op.bin", np.float32).reshape(-1, 3)
@@ -51,3 +51,8 @@ def test_dropin_matches_python_api(tmp_path):
     for _ in range(3):
         pp, pn = ap.project(len(q), np.zeros_like(q), np.zeros_like(q), 1, K)
     assert np.array_equal(op, pp) and np.array_equal(on, pn)
+    cells = np.fromfile(tmp_path / "cells.bin", np.int32).reshape(-1, 4)
+    gen = A.OctreeGenerator(ap, K)
+    total = []
+    assert np.array_equal(cells, gen.generate(4, total))
+    assert f"cells {len(cells)} total {len(total[0])}" in r.stdout
