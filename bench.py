@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 PEAK_HBM_GBS = 8000.0
+SERVE_BATCH, SERVE_CHUNK, SERVE_INFLIGHT = 262144, 65536, 4
 
 
 def log(*a):
@@ -63,6 +64,15 @@ def algorithmic_work(counts, m, kern):
     return float(flops), float(trans)
 
 
+def ball_population(w, sample=4000):
+    """Mean number of points in a query's ball (radius 3 sigma_0), from a seeded query sample."""
+    from scipy.spatial import cKDTree
+    r = 3.0 * w["kernel"]["sigmas"][0] if w["ball_radius"] < 0 else w["ball_radius"]
+    q = w["queries"][np.random.default_rng(1).choice(len(w["queries"]), min(sample, len(w["queries"])), replace=False)]
+    t = cKDTree(w["points"])
+    return float(np.mean([len(x) for x in t.query_ball_point(q, r)]))
+
+
 def cpu_baseline(w, kern, cutoff, budget_s=15.0):
     """The reference's CPU projection (APSSOctree::projectCPU with OpenMP over queries,
     scene.cpp:982-986), REF_CPU semantics, one step per query, on a bounded sample."""
@@ -74,7 +84,8 @@ def cpu_baseline(w, kern, cutoff, budget_s=15.0):
     tree = bind.RefTree(w["points"], w["normals"])
     build_s = time.time() - t0
     k = bind.make_kernel(**kern)
-    prm = bind.make_params(cutoff=cutoff, semantics="ref_cpu")
+    prm = bind.make_params(cutoff=cutoff, semantics="ref_cpu", mode=w.get("mode", "mlod"), knn=w.get("knn", 20),
+                           ball_radius=w.get("ball_radius", 0.1))
     q = w["queries"]
     n0 = min(len(q), 4000)
     t0 = time.time()
@@ -85,8 +96,22 @@ def cpu_baseline(w, kern, cutoff, budget_s=15.0):
     tree.project(q[:ns], k, prm, n_iter=1, nthreads=cores)
     dt = time.time() - t0
     return {"value": ns / dt, "unit": "projected pts/s", "cores": cores, "kind": "reference",
-            "sample": f"first {ns} of {len(q)} queries, 1 step each, reference APSSOctree::projectCPU (REF_CPU) "
-                      f"with OpenMP on {cores} host threads; reference octree build {build_s:.1f} s untimed"}
+            "sample": f"first {ns} of {len(q)} queries, 1 step each, reference APSSOctree::projectCPU (REF_CPU, "
+                      f"{w.get('mode', 'mlod')}) with OpenMP on {cores} host threads; reference octree build "
+                      f"{build_s:.1f} s untimed"}
+
+
+def calibrate_copy():
+    """A kernel of known traffic for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: out = 2 * in over
+    128 Mi floats, i.e. 512 MiB read and 512 MiB written per dispatch, three dispatches."""
+    import torch
+    n = 128 << 20
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 dispatches, {n * 4} bytes read and written each")
 
 
 def traffic_from_profile(kernel_name, workload):
@@ -113,7 +138,7 @@ def serve_measure(n_clouds=64, seed_rate=2000.0):
     trees = [APSSOctree().build(p, n) for p, n in clouds]
     npts = sum(len(c[0]) for c in clouds)
     log(f"serve: {len(clouds)} clouds, {npts / 1e6:.1f}M points resident, ready in {time.time() - t0:.1f}s")
-    srv = SV.Server(max_batch=524288, inflight=3)
+    srv = SV.Server(max_batch=SERVE_BATCH, chunk=SERVE_CHUNK, inflight=SERVE_INFLIGHT)
     # capacity: a Poisson request mix submitted at once (offered rate far above service)
     cap, _, _ = SV.run_load(srv, trees, clouds, rate=1e6, seconds=1.0, max_requests=400, seed=11, log=log)
     mean_pts = cap["projected_pts"] / max(cap["requests"], 1)
@@ -123,7 +148,8 @@ def serve_measure(n_clouds=64, seed_rate=2000.0):
     sus, _, _ = SV.run_load(srv, trees, clouds, rate=sus_rate, seconds=3.0, seed=13, log=log)
     srv.close()
     keys = ("requests", "offered_rate", "pts_per_s", "p50_ms", "p99_ms", "p50_100k_ms", "p99_100k_ms", "n_100k", "wall_s")
-    return {"clouds": len(clouds), "resident_points": npts, "max_batch": 524288, "inflight": 3,
+    return {"clouds": len(clouds), "resident_points": npts, "max_batch": SERVE_BATCH, "chunk": SERVE_CHUNK,
+            "inflight": SERVE_INFLIGHT,
             "capacity_pts_per_s": cap["pts_per_s"], "capacity_req_per_s": cap_rate,
             "at_2000_req_s": {k: spec[k] for k in keys},
             "at_70pct_capacity": {k: sus[k] for k in keys},
@@ -135,7 +161,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="superquadric8m")
+    ap.add_argument("--workload", default="superquadric8m",
+                    choices=["superquadric8m", "sphere100k", "torus1m", "c5_knn", "c5_ball"])
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--precision", default="fp32c", choices=["fp32", "fp32c", "fp64"])
     ap.add_argument("--semantics", default="ref_gpu", choices=["ref_gpu", "ref_cpu"])
@@ -144,7 +171,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-clouds", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain elementwise copy of 512 MiB three times (to scale the HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only.  The projection path would shard by "
               "replicating the octree and splitting the Morton-ordered queries across GPUs; that is not "
@@ -177,6 +209,11 @@ def main():
         f"{info['device_bytes'] / 2**20:.0f} MiB")
 
     K = A.Kernel(**w["kernel"])
+    mode = w.get("mode", "mlod")
+    if mode == "knn":
+        tree.knnMode(True, w["knn"])
+    elif mode == "ball":
+        tree.ballMode(True, w["ball_radius"])
     forms = [args.precision] + ([] if args.no_forms else [f for f in ("fp32", "fp64") if f != args.precision])
     if "fp64" in forms:
         forms = [f for f in forms if f != "fp64"] + ["fp64"]
@@ -189,7 +226,7 @@ def main():
         ap_.set_semantics(args.semantics)
         ap_.set_precision(form)
         ap_.copyPointsToGPU_device(dQ.ptr.value, m, stream=stream.ptr)
-        if counts is None:
+        if counts is None and mode == "mlod":
             counts = ap_.count_visits(stream=stream.ptr)
             log(f"counters (one step): {counts}")
         for _ in range(args.warmup):
@@ -225,7 +262,13 @@ def main():
 
     head = results[args.precision]
     mean_launch_s = float(np.mean(head["launch_ms"])) / 1e3
-    flops, trans = algorithmic_work(counts, m, w["kernel"])
+    if mode == "mlod":
+        flops, trans = algorithmic_work(counts, m, w["kernel"])
+    else:
+        nb = float(w["knn"]) if mode == "knn" else ball_population(w)
+        counts = {"neighbours_per_query": nb}
+        wf, wt = weight_cost(w["kernel"])
+        flops, trans = float(m) * (nb * (27 + wf) + 70), float(m) * nb * wt
     achieved = (flops + 8.0 * trans) / mean_launch_s / 1e12
     comp_bytes = 64.0 * info["n_nodes"] + 32.0 * n + 48.0 * m
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -240,7 +283,7 @@ def main():
     del tree, dP, dN, dQ
     hiprt.device_synchronize()
     serving = None
-    if not args.no_serve:
+    if not args.no_serve and args.workload == "superquadric8m" and args.scale == 1.0:
         try:
             serving = serve_measure(args.serve_clouds)
         except Exception as ex:
@@ -267,8 +310,9 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp32c": "fp32-compensated+fp64-fit", "fp32": "fp32", "fp64": "fp64"}[args.precision],
         "data": "synthetic (seeded superquadric cloud and offset queries)",
-        "config": {"workload": args.workload if args.scale == 1.0 else f"{args.workload}@{args.scale}",
-                   "points": n, "queries": m, "kernel": "multi-Gaussian S=%d" % len(w["kernel"].get("sigmas", [])),
+        "config": {"workload": args.workload if args.scale == 1.0 else f"{args.workload}@{args.scale}", "mode": mode,
+                   "points": n, "queries": m, "kernel": {0: "Gaussian", 1: "rational", 2: "multi-Gaussian S=%d" % len(w["kernel"].get("sigmas", []))}[
+                       w["kernel"]["type"]],
                    "cutoff": w["cutoff"], "min_depth": 0, "semantics": args.semantics, "precision": args.precision,
                    "octree_nodes": info["n_nodes"], "octree_depth": info["depth"], "build_ms": build_ms,
                    "parallelism": "single GPU"},

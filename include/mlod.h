@@ -121,6 +121,18 @@ int mlod_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, floa
 /* Copies the current positions / normals of the first n queries to host arrays. */
 int mlod_queries_download(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, void* stream);
 
+/* APSS::projectCPU (GPUUtils.h:33-34; APSSOctree::projectCPU, CPUUtils.cpp:621-636), batched:
+ * n_iter steps of m host points (m x 3) with the tree's current settings, results to out_pts /
+ * out_nmls (host, m x 3).  The uploaded queries are left untouched.  The reference's CPU function
+ * is selected with MLOD_SEM_REF_CPU in mlod_set_params. */
+int mlod_project_points(mlod_tree* t, const float* xyz, uint32_t m, uint32_t n_iter, float step_max, float* out_pts,
+                        float* out_nmls, void* stream);
+
+/* APSS::reorganizePoints (GPUUtils.cu:436-450): for i = 0 .. n-1 in order, the uploaded queries
+ * invalid[i] and end_valid - 1 - i (caller's indices) exchange positions; normals stay with their
+ * index, as in the reference (which exchanges positions only). */
+int mlod_queries_reorganize(mlod_tree* t, const uint32_t* invalid, uint32_t n, uint32_t end_valid, void* stream);
+
 /* Octree dump in BFS order (the flattening of Fastapss copy(), Fastapss.cu:97-150, via
  * getRelations, CPUUtils.cpp:213-234), for parity checks.  Any pointer may be NULL.
  * leaf_idx receives, for each leaf in BFS order, its point indices in input order. */
@@ -136,26 +148,29 @@ int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out4, void* stream);
 /* ---------------------------------------------------------------------------------------
  * Serving.  No counterpart in the reference (its harness projects one batch at a time with
  * blocking calls, scene.cpp:916-966).  A server keeps registered octrees resident, takes
- * projection requests from any thread, packs pending requests oldest-first into batches of at
- * most max_batch_queries queries and keeps up to `inflight` batches on the device at once
- * (continuous batching).  Each query gives the same result alone or in any batch.
- * Registered trees must outlive the server; request buffers must stay valid until the
+ * projection requests from any thread, cuts them into chunks of at most chunk_queries queries,
+ * packs pending chunks into batches of at most max_batch_queries queries, earliest deadline first
+ * (arrival time plus the request's own projected-point count at a nominal rate: equal requests in
+ * arrival order, small ones not stuck behind large ones), and keeps up to `inflight` batches on
+ * the device at once (continuous batching).  Each query gives the same result alone or in any
+ * batch.  Registered trees must outlive the server; request buffers must stay valid until the
  * request's completion is polled. */
 typedef struct mlod_server mlod_server;
 typedef struct {
   uint64_t ticket;
-  double latency_ms;   /* submit -> results copied to the caller's buffers */
-  double queue_ms;     /* submit -> its batch launched */
+  double latency_ms;   /* submit -> all results copied to the caller's buffers */
+  double queue_ms;     /* submit -> first chunk's batch launched */
   double completed_s;  /* completion time since server creation */
 } mlod_server_done;
 
-int mlod_server_create(uint32_t max_batch_queries, uint32_t inflight, int32_t semantics, int32_t precision,
-                       mlod_server** out);
+/* chunk_queries = 0: max_batch_queries / 4 (rounded to 128) */
+int mlod_server_create(uint32_t max_batch_queries, uint32_t chunk_queries, uint32_t inflight, int32_t semantics,
+                       int32_t precision, mlod_server** out);
 int mlod_server_destroy(mlod_server* s);
 int mlod_server_add_tree(mlod_server* s, mlod_tree* t, uint32_t* tree_id);
-/* n_iter projection steps of m queries (host, m x 3) on tree tree_id with kernel k; results go
- * to out_pts / out_nmls (host, m x 3, either may be NULL).  step_max < 0 => the tree's radius.
- * Multi-Gaussian kernels take at most 8 sigmas here. */
+/* n_iter projection steps of m queries (host, m x 3, any m >= 1) on tree tree_id with kernel k;
+ * results go to out_pts / out_nmls (host, m x 3, either may be NULL).  step_max < 0 => the tree's
+ * radius.  Multi-Gaussian kernels take at most 8 sigmas here. */
 int mlod_server_submit(mlod_server* s, uint32_t tree_id, const mlod_kernel* k, float cutoff, uint32_t min_depth,
                        uint32_t n_iter, float step_max, const float* queries, uint32_t m, float* out_pts,
                        float* out_nmls, uint64_t* ticket);

@@ -7,13 +7,14 @@
 // Vector arguments take any 3-float type laid out as x, y, z (glm::vec3 included).
 //
 // Differences: errors throw mlod::Error instead of exit() (gpuErrchk, GPUUtils.cu:32-41);
-// the octree is built on the GPU by APSSOctree::build; projectCPU is not provided (the
-// product has no CPU path); set_semantics / set_precision choose the reference function
-// and the arithmetic form (see mlod.h).
+// the octree is built on the GPU by APSSOctree::build; projectCPU computes the reference's CPU
+// function on the GPU (the product has no CPU path); set_semantics / set_precision choose the
+// reference function and the arithmetic form (see mlod.h).
 #ifndef MLOD_APSS_HPP
 #define MLOD_APSS_HPP
 
 #include <cmath>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -164,6 +165,48 @@ class APSS {
             n_iterations, kernel, stepMaxSize);
   }
 
+  // APSS::projectCPU (GPUUtils.h:33-34): the reference's CPU function (REF_CPU semantics,
+  // APSSOctree::projectCPU, CPUUtils.cpp:621-636) for one point, computed on the GPU without
+  // touching the uploaded queries.  Batch callers (scene::projectPointsOnCPU) should pass all
+  // their points to projectPoints instead of looping over this.
+  template <class V>
+  void projectCPU(const V& qStart, V& outputPoint, V& outputNormal, unsigned n_iterations,
+                  const Kernel& kernel = Kernel(), float stepMaxSize = -1.f) const {
+    projectPoints(1, &qStart, &outputPoint, &outputNormal, n_iterations, kernel, stepMaxSize, MLOD_SEM_REF_CPU);
+  }
+  template <class V>
+  void projectPoints(unsigned n, const V* points, V* outputPoints, V* outputNormals, unsigned n_iterations,
+                     const Kernel& kernel, float stepMaxSize = -1.f, int semantics = -1) const {
+    (void)kernel;
+    sync(semantics < 0 ? m_semantics : semantics);
+    check(mlod_project_points(m_apssOctree->handle(), fptr(points), n, n_iterations, stepMaxSize,
+                              outputPoints ? fptr(outputPoints) : nullptr, outputNormals ? fptr(outputNormals) : nullptr,
+                              m_stream));
+  }
+
+  // APSS::reorganizePoints (GPUUtils.cu:436-450)
+  void reorganizePoints(const std::vector<unsigned>* invalidPts, unsigned endValid) {
+    static_assert(sizeof(unsigned) == sizeof(uint32_t), "unsigned is 32-bit");
+    check(mlod_queries_reorganize(m_apssOctree->handle(), reinterpret_cast<const uint32_t*>(invalidPts->data()),
+                                  (uint32_t)invalidPts->size(), endValid, m_stream));
+  }
+
+  // APSS::printApss (GPUUtils.cu:426-434)
+  void printApss() const {
+    const mlod_tree_info i = m_apssOctree->info();
+    std::printf("m_numberOfNodes: %u\nm_scalingProtectionSphere: %g\nm_minDepth: %u\n", i.n_nodes,
+                (double)m_scalingProtectionSphere, m_minDepth);
+  }
+  // APSS::stop (GPUUtils.cu:394-397) stops the CUDA profiler; there is nothing to stop here.
+  void stop() {}
+  // APSS::eraseAndRestart (GPUUtils.cu:399-404): like the reference, the APSS owns and deletes
+  // its octree here and continues with a new, empty one.
+  void eraseAndRestart() {
+    eraseFromGPU();
+    delete m_apssOctree;
+    m_apssOctree = new APSSOctree();
+  }
+
   // Fastapss::copyToGPU (Fastapss.cu:457-484): the tree is already resident; installs the kernel.
   virtual void copyToGPU(const Kernel& kernel, unsigned nbOfVectors) {
     (void)nbOfVectors;
@@ -211,9 +254,9 @@ class APSS {
     m_scalingProtectionSphere = octree->getScalingProtectionSphere();
     m_minDepth = octree->getMinDepth();
   }
-  void sync() const {
+  void sync(int semantics = -1) const {
     mlod_tree* t = m_apssOctree->handle();
-    check(mlod_set_params(t, m_scalingProtectionSphere, m_minDepth, m_semantics, m_precision));
+    check(mlod_set_params(t, m_scalingProtectionSphere, m_minDepth, semantics < 0 ? m_semantics : semantics, m_precision));
     int mode = m_mode;
     if (mode == MLOD_MODE_MLOD && m_apssOctree->m_knnMode) mode = MLOD_MODE_KNN;
     if (mode == MLOD_MODE_MLOD && m_apssOctree->m_ballMode) mode = MLOD_MODE_BALL;

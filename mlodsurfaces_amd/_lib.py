@@ -65,13 +65,15 @@ SIGNATURES = {
     "mlod_queries_download": (C.c_int, [_vp, C.c_uint32, _f, _f, _vp]),
     "mlod_tree_dump": (C.c_int, [_vp, _i, _f, _f, _f, _i, _i, _i, _i, _u, _u, _f, _f]),
     "mlod_count_visits": (C.c_int, [_vp, C.c_uint32, _u64, _vp]),
-    "mlod_server_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
+    "mlod_server_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, C.POINTER(_vp)]),
     "mlod_server_destroy": (C.c_int, [_vp]),
     "mlod_server_add_tree": (C.c_int, [_vp, _vp, _u]),
     "mlod_server_submit": (C.c_int, [_vp, C.c_uint32, C.POINTER(KernelC), C.c_float, C.c_uint32, C.c_uint32, C.c_float,
                                      _vp, C.c_uint32, _vp, _vp, _u64]),
     "mlod_server_poll": (C.c_int, [_vp, C.POINTER(ServerDoneC), C.c_uint32, _u, C.c_double]),
     "mlod_server_stats": (C.c_int, [_vp, _u64]),
+    "mlod_project_points": (C.c_int, [_vp, _f, C.c_uint32, C.c_uint32, C.c_float, _f, _f, _vp]),
+    "mlod_queries_reorganize": (C.c_int, [_vp, _u, C.c_uint32, C.c_uint32, _vp]),
     "mlod_generator_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "mlod_generator_destroy": (C.c_int, [_vp]),
     "mlod_generate": (C.c_int, [_vp, C.c_uint32, C.c_int32, _vp]),

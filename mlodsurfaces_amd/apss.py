@@ -326,6 +326,38 @@ class APSS:
                                  _fp(op) if op is not None else None, _fp(on) if on is not None else None, stream))
         return op, on
 
+    def project_points(self, points, n_iterations=1, stepMaxSize=-1.0, stream=None):
+        """Projects host points (m, 3) with the current settings without touching the uploaded
+        queries; returns (points, normals)."""
+        q = _f32(points, "points")
+        self._sync_params()
+        op = np.zeros((len(q), 3), np.float32)
+        on = np.zeros((len(q), 3), np.float32)
+        check(lib().mlod_project_points(self.m_apssOctree.handle, _fp(q), len(q), int(n_iterations), float(stepMaxSize),
+                                        _fp(op), _fp(on), stream))
+        return op, on
+
+    def projectCPU(self, qStart, n_iterations=1, kernel=None, stepMaxSize=-1.0, stream=None):
+        """APSS::projectCPU (GPUUtils.h:33-34): the reference's CPU function (REF_CPU semantics,
+        APSSOctree::projectCPU, CPUUtils.cpp:621-636), computed on the GPU for one point (3,) or
+        a batch (m, 3); returns (points, normals) of the same shape."""
+        q = np.asarray(qStart, np.float32)
+        single = q.ndim == 1
+        sem = self.semantics
+        self.semantics = _lib.SEM_REF_CPU
+        try:
+            op, on = self.project_points(q.reshape(-1, 3), n_iterations, stepMaxSize, stream)
+        finally:
+            self.semantics = sem
+        return (op[0], on[0]) if single else (op, on)
+
+    def reorganizePoints(self, invalidPts, endValid, stream=None):
+        """APSS::reorganizePoints (GPUUtils.cu:436-450): uploaded queries invalidPts[i] and
+        endValid - 1 - i exchange positions, in order."""
+        inv = np.ascontiguousarray(invalidPts, np.uint32)
+        check(lib().mlod_queries_reorganize(self.m_apssOctree.handle, inv.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                            len(inv), int(endValid), stream))
+
     def download(self, n=None, stream=None):
         n = self._m if n is None else int(n)
         op = np.zeros((n, 3), np.float32)

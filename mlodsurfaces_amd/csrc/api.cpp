@@ -184,6 +184,17 @@ int mlod_queries_erase(mlod_tree* t) {
   return MLOD_OK;
 }
 
+namespace {
+// n_iter steps of the first n uploaded queries in the tree's neighbourhood mode.  kNN / ball
+// APSS take no step bound: their fit is called with q aliased to the output
+// (CPUUtils.cpp:735), which makes the clamp a no-op.
+int project_uploaded(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s) {
+  if (!n_iter) return MLOD_OK;
+  return (t->mode == MLOD_MODE_KNN || t->mode == MLOD_MODE_BALL) ? launch_neighbors(t, n, n_iter, s)
+                                                                 : launch_project(t, n, n_iter, step_max, s);
+}
+}  // namespace
+
 int mlod_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, float* out_pts, float* out_nmls, void* stream) {
   if (!t) return fail(MLOD_ERR_INVALID, "tree is NULL");
   if (n == 0) return MLOD_OK;  // Fastapss::project returns on an empty batch (Fastapss.cu:441-445)
@@ -191,16 +202,35 @@ int mlod_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, floa
   if (n > t->qs.m) return fail(MLOD_ERR_INVALID, "n exceeds the number of uploaded queries");
   if (step_max < 0.f) step_max = t->radius;  // Fastapss.cu:448-449
   hipStream_t s = (hipStream_t)stream;
-  int r = MLOD_OK;
-  if (n_iter) {
-    // kNN / ball APSS take no step bound: their fit is called with q aliased to the output
-    // (CPUUtils.cpp:735), which makes the clamp a no-op.
-    r = (t->mode == MLOD_MODE_KNN || t->mode == MLOD_MODE_BALL) ? launch_neighbors(t, n, n_iter, s)
-                                                                 : launch_project(t, n, n_iter, step_max, s);
-  }
-  if (r) return r;
+  if (int r = project_uploaded(t, n, n_iter, step_max, s)) return r;
   if (out_pts || out_nmls) return queries_get(t, n, out_pts, out_nmls, s);
   return MLOD_OK;
+}
+
+int mlod_project_points(mlod_tree* t, const float* xyz, uint32_t m, uint32_t n_iter, float step_max, float* out_pts,
+                        float* out_nmls, void* stream) {
+  if (!t) return fail(MLOD_ERR_INVALID, "tree is NULL");
+  if (m == 0) return MLOD_OK;
+  if (!xyz) return fail(MLOD_ERR_INVALID, "points are NULL");
+  if (!t->kernel_set) return fail(MLOD_ERR_STATE, "no kernel set (mlod_set_kernel)");
+  if (step_max < 0.f) step_max = t->radius;
+  hipStream_t s = (hipStream_t)stream;
+  // the uploaded queries are set aside and restored: these points use their own buffers
+  const QueryState saved = t->qs;
+  t->qs = QueryState{};
+  int r = queries_set(t, xyz, m, false, s);
+  if (!r) r = project_uploaded(t, m, n_iter, step_max, s);
+  if (!r && (out_pts || out_nmls)) r = queries_get(t, m, out_pts, out_nmls, s);
+  (void)hipStreamSynchronize(s);
+  queries_free(t);
+  t->qs = saved;
+  return r;
+}
+
+int mlod_queries_reorganize(mlod_tree* t, const uint32_t* invalid, uint32_t n, uint32_t end_valid, void* stream) {
+  if (!t) return fail(MLOD_ERR_INVALID, "tree is NULL");
+  if (n && !invalid) return fail(MLOD_ERR_INVALID, "invalid list is NULL");
+  return queries_reorganize(t, invalid, n, end_valid, (hipStream_t)stream);
 }
 
 int mlod_queries_download(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, void* stream) {

@@ -191,5 +191,6 @@ int launch_neighbors(mlod_tree* t, uint32_t n, uint32_t n_iter, hipStream_t s);
 int queries_set(mlod_tree* t, const float* xyz, uint32_t m, bool device_src, hipStream_t s);
 int queries_get(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, hipStream_t s);
 void queries_free(mlod_tree* t);
+int queries_reorganize(mlod_tree* t, const uint32_t* invalid, uint32_t n, uint32_t end_valid, hipStream_t s);
 
 }  // namespace mlod

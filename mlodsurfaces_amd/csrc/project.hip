@@ -303,26 +303,27 @@ __device__ __forceinline__ T tratio(T a, T b) {  // a / b
 
 template <class T>
 struct LevelState {
-  uint32_t (*first)[PBS];
-  uint8_t (*mask)[PBS];
-  T (*mul)[PBS];
-  T (*dist)[PBS];
+  uint32_t (*node)[PBS];  // node whose near children wait
+  uint8_t (*mask)[PBS];   // its near children still to expand
+  T (*mul)[PBS];          // its multiplier
 };
 
 // Flattened traversal.  The lane's current node (expanded: near, or above minDepth) has its
 // children tested one per loop iteration; when they are done the node's own (1 - b_c) share
-// is added and the next pending near child anywhere on the lane's path becomes the current
-// node.  Either way an iteration handles exactly one node X: it loads X, computes its distance
-// and its blend factor against its parent, then either classifies X as a child (far / leaf /
-// near) or makes it the new current node, and ends in one shared node contribution.  Lanes of
-// a wavefront therefore advance by one node per iteration whatever their position in their own
-// tree walk, instead of waiting for the widest child loop of the wavefront at every level.
+// is added and its first near child becomes the current node.  Either way an iteration handles
+// exactly one node X: it loads X, computes its distance and its blend factor against its
+// parent, then either classifies X as a child (far / leaf / near) or makes it the current node,
+// and ends in one shared node contribution.  Lanes of a wavefront therefore advance by one node
+// per iteration whatever their position in their own tree walk, instead of waiting for the
+// widest child loop of the wavefront at every level.
 //
 // Each iteration issues all its global loads at once (X's geometry and header, and the moments
 // of the node it may contribute: X itself when testing, the finished node when moving on), so
-// one memory round trip covers the iteration.  Levels with near children still to expand are
-// kept in a per-lane bit set; the per-level state (first child, remaining near children,
-// multiplier, parent distance) lives in LDS and is only written when a second near child waits.
+// one memory round trip covers the iteration.  A finished node with more than one near child
+// leaves {node, remaining near children, multiplier} in LDS for its depth (9 B per level and
+// lane, a per-lane bit set marks the levels in use); when a lane's path runs out, the deepest
+// such node is reloaded and becomes the current node again with its remaining near children,
+// its distance term recomputed from its geometry (the same operations, so the same bits).
 template <int FORM, int SEM, int KT, bool CNT, class KP>
 __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv,
                          V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff, uint32_t minDepth,
@@ -349,9 +350,11 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
   T shareScale = SEM == MLOD_SEM_REF_CPU ? tratio<FORM>(T(1), (T)nc) : tratio<FORM>(T(1), (T)Ai);
   bool active = true;
   while (active) {
-    // --- pick X: the next child of the current node, or the next pending node to expand
+    // --- pick X: the next child of the current node, the next near node to expand, or a
+    // node with near children still waiting, to restore
     const bool testing = k < nc;
-    uint32_t x = 0;
+    bool restore = false;
+    uint32_t x = 0, rmask = 0;
     T mp = T(0), dp = T(-1);
     bool ownContrib = false;
     const uint32_t ownNode = node;
@@ -364,15 +367,14 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
     } else {
       ownContrib = blendNode && shareSum != T(0);
       ownF = m * shareSum;
-      if constexpr (CNT) ct.nearN += blendNode ? 1u : 0u;
+      if constexpr (CNT) ct.nearN += (blendNode && shareSum != T(0)) ? 1u : 0u;
       if (mask) {
         // the finished node's first near child is next; the rest wait at this level
         const uint32_t rest = mask & (mask - 1u);
         if (rest) {
-          st.first[lvl][tid] = first;
+          st.node[lvl][tid] = node;
           st.mask[lvl][tid] = (uint8_t)rest;
           st.mul[lvl][tid] = m;
-          st.dist[lvl][tid] = blendNode ? distP : T(-1);
           pend |= 1u << lvl;
         }
         x = first + (uint32_t)__builtin_ctz(mask);
@@ -381,14 +383,12 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
         lvl = lvl + 1;
       } else if (pend) {
         const int L = 31 - __builtin_clz(pend);
-        const uint32_t pm = st.mask[L][tid];
-        const uint32_t rest = pm & (pm - 1u);
-        st.mask[L][tid] = (uint8_t)rest;
-        if (!rest) pend &= ~(1u << L);
-        x = st.first[L][tid] + (uint32_t)__builtin_ctz(pm);
+        pend &= ~(1u << L);
+        x = st.node[L][tid];
+        rmask = st.mask[L][tid];
         mp = st.mul[L][tid];
-        dp = st.dist[L][tid];
-        lvl = L + 1;
+        restore = true;
+        lvl = L;
       } else {
         active = false;
       }
@@ -414,7 +414,7 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
         const T dc = fmax(dxq - rc, T(0));
         beta = blend(tratio<FORM>(dp, dp + dc));
       }
-      const T mx = mp * beta;
+      const T mx = restore ? mp : mp * beta;
       const float Ax = __uint_as_float(hx.z);
       if (testing) {
         ++k;
@@ -437,12 +437,13 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
         At = Ax;
         f = mx;
       } else {
-        // X becomes the current node
+        // X becomes the current node: a new one (children to test) or a restored one
+        // (children tested, near children rmask still to expand, own share already added)
         node = x;
         first = hx.x;
         nc = hdr_count(hx.y);
-        k = 0;
-        mask = 0;
+        k = restore ? nc : 0u;
+        mask = rmask;
         blendNode = hdr_depth(hx.y) >= minDepth;
         distP = fabs(dxq - rc);
         m = mx;
@@ -563,15 +564,14 @@ __global__ __launch_bounds__(PBS) void k_project(TreeView tv, KParams kp, float4
                                                  uint32_t nIter, float stepMax, float cutoff, uint32_t minDepth,
                                                  unsigned long long* __restrict__ counters) {
   using T = typename FormT<FORM>::T;
-  __shared__ uint32_t sFirst[kLevels][PBS];
+  __shared__ uint32_t sNode[kLevels][PBS];
   __shared__ uint8_t sMask[kLevels][PBS];
   __shared__ T sMul[kLevels][PBS];
-  __shared__ T sDist[kLevels][PBS];
   const int tid = threadIdx.x;
   const uint32_t i = xcd_block() * PBS + tid;
   if (i >= m) return;
   if (limit < m && perm[i] >= limit) return;
-  const LevelState<T> st{sFirst, sMask, sMul, sDist};
+  const LevelState<T> st{sNode, sMask, sMul};
   const float4 q4 = q[i];
   V3<double> qd{q4.x, q4.y, q4.z};
   float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -654,10 +654,9 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
                                                        float4* __restrict__ q, float4* __restrict__ qn, uint32_t blk0) {
   static_assert(PBS == kBatchWG, "one workgroup per aligned request block");
   using T = typename FormT<FORM>::T;
-  __shared__ uint32_t sFirst[kLevels][PBS];
+  __shared__ uint32_t sNode[kLevels][PBS];
   __shared__ uint8_t sMask[kLevels][PBS];
   __shared__ T sMul[kLevels][PBS];
-  __shared__ T sDist[kLevels][PBS];
   const int tid = threadIdx.x;
   const uint32_t blk = blk0 + xcd_block();
   const uint32_t r = __builtin_amdgcn_readfirstlane(blockReq[blk]);
@@ -668,7 +667,7 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
   const WCoef w = R.w;
   const float cutoff = R.cutoff, stepMax = R.stepMax;
   const uint32_t minDepth = R.minDepth, nIter = R.nIter;
-  const LevelState<T> st{sFirst, sMask, sMul, sDist};
+  const LevelState<T> st{sNode, sMask, sMul};
   const float4 q4 = q[i];
   V3<double> qd{q4.x, q4.y, q4.z};
   float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);

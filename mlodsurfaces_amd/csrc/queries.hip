@@ -68,6 +68,34 @@ __global__ __launch_bounds__(BS) void k_scatter(const float4* __restrict__ src, 
   out[3 * (uint64_t)d + 2] = v.z;
 }
 
+// inverse of the sort permutation: caller index -> sorted slot
+__global__ __launch_bounds__(BS) void k_invperm(const uint32_t* __restrict__ perm, uint32_t m, uint32_t* __restrict__ inv) {
+  const uint32_t j = blockIdx.x * BS + threadIdx.x;
+  if (j < m) inv[perm[j]] = j;
+}
+
+// APSS::reorganizePoints (GPUUtils.cu:413-450): for i = 0, 1, ..., n - 1 in order, the caller's
+// queries invalid[i] and endValid - 1 - i exchange positions; normals stay with their index.
+// Positions are exchanged by exchanging the two slots' owners (perm), which keeps the Morton
+// order of the slots, and the two slots' normals are exchanged back.  One lane runs the list in
+// order, so overlapping pairs resolve as in a sequential loop.  Indices were checked on the host.
+__global__ void k_reorganize(const uint32_t* __restrict__ invalid, uint32_t n, uint32_t endValid,
+                             uint32_t* __restrict__ perm, uint32_t* __restrict__ inv, float4* __restrict__ qn) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t a = invalid[i], b = endValid - 1u - i;
+    if (a == b) continue;
+    const uint32_t sa = inv[a], sb = inv[b];
+    perm[sa] = b;
+    perm[sb] = a;
+    inv[a] = sb;
+    inv[b] = sa;
+    const float4 t = qn[sa];
+    qn[sa] = qn[sb];
+    qn[sb] = t;
+  }
+}
+
 }  // namespace
 
 void queries_free(mlod_tree* t) {
@@ -144,6 +172,38 @@ int queries_get(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, hipSt
     MLOD_HIP(hipMemcpyAsync(out_nmls, qs.stage, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
     MLOD_HIP(hipStreamSynchronize(s));
   }
+  return MLOD_OK;
+}
+
+}  // namespace mlod
+
+namespace mlod {
+
+int queries_reorganize(mlod_tree* t, const uint32_t* invalid, uint32_t n, uint32_t end_valid, hipStream_t s) {
+  QueryState& qs = t->qs;
+  if (end_valid > qs.m) return fail(MLOD_ERR_INVALID, "end_valid exceeds the number of uploaded queries");
+  if (n > end_valid) return fail(MLOD_ERR_INVALID, "more invalid points than valid ones");
+  for (uint32_t i = 0; i < n; ++i)
+    if (invalid[i] >= qs.m) return fail(MLOD_ERR_INVALID, "invalid point index out of range");
+  if (n == 0) return MLOD_OK;
+  uint32_t *dInv = nullptr, *dList = nullptr;
+  if (hipMalloc((void**)&dInv, (size_t)qs.m * 4) != hipSuccess || hipMalloc((void**)&dList, (size_t)n * 4) != hipSuccess) {
+    (void)hipFree(dInv);
+    return fail(MLOD_ERR_OOM, "reorganize buffers");
+  }
+  hipError_t e = hipMemcpyAsync(dList, invalid, (size_t)n * 4, hipMemcpyHostToDevice, s);
+  if (!e) {
+    k_invperm<<<(qs.m + BS - 1) / BS, BS, 0, s>>>(qs.perm, qs.m, dInv);
+    e = hipGetLastError();
+  }
+  if (!e) {
+    k_reorganize<<<1, 1, 0, s>>>(dList, n, end_valid, qs.perm, dInv, qs.qn);
+    e = hipGetLastError();
+  }
+  if (!e) e = hipStreamSynchronize(s);
+  (void)hipFree(dInv);
+  (void)hipFree(dList);
+  if (e) return hip_fail(e, "reorganize", __FILE__, __LINE__);
   return MLOD_OK;
 }
 

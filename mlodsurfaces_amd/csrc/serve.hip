@@ -2,11 +2,15 @@
 //
 // The reference has no server: its harness projects one query batch at a time with blocking
 // calls (scene.cpp:916-966).  Here clients submit projection requests (tree, kernel,
-// parameters, queries, n_iter) from any thread; a worker thread packs the pending requests,
-// oldest first, into batches bounded by a slot budget (device memory per batch is fixed at
-// creation), and keeps up to `inflight` batches on separate streams, so requests that arrive
-// while others are on the GPU join the next batch as soon as a batch slot frees, and the
-// batches' copies and kernels overlap on the device.
+// parameters, queries, n_iter) from any thread.  A request is cut into chunks of at most
+// `chunk` queries; a worker thread packs pending chunks into batches bounded by a slot budget
+// (device memory per batch is fixed at creation) and keeps up to `inflight` batches on separate
+// streams, so requests that arrive while others are on the GPU join the next batch as soon as a
+// batch slot frees, and the batches' copies and kernels overlap on the device.  Chunks are taken
+// earliest deadline first, the deadline being the arrival time plus the request's own service
+// time at the nominal rate (kNominalPtsPerMs projected points per ms): requests of equal size
+// are served in arrival order, and a small request that arrives behind a large one does not
+// wait for all of it.
 //
 // One batch on its stream: pinned host staging -> device copy; per-slot Morton key in the
 // request's tree box with the request index above it; radix sort (so each wavefront walks one
@@ -27,6 +31,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <queue>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -38,7 +43,8 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 constexpr int BS = 256;
-constexpr uint32_t kMaxReqPerBatch = 1024;  // request index fits 10 bits above the 30-bit Morton code
+constexpr uint32_t kMaxReqPerBatch = 1024;  // chunk index fits 10 bits above the 30-bit Morton code
+constexpr double kNominalPtsPerMs = 2.0e5;  // deadline slack per projected point (200 M pts/s)
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   v &= 0x3ffu;
@@ -109,7 +115,8 @@ __global__ __launch_bounds__(BS) void k_batch_scatter(const float4* __restrict__
   out[3 * ((uint64_t)cap + d) + 2] = n.z;
 }
 
-struct SReq {
+// One request as submitted.
+struct Request {
   uint64_t ticket;
   uint32_t tree;
   mlod_kernel kernel;
@@ -121,7 +128,22 @@ struct SReq {
   float* outP;
   float* outN;
   uint32_t m;
-  clk::time_point submitted;
+  uint32_t chunksLeft;  // chunks not yet completed (worker thread only)
+  clk::time_point submitted, launched;
+};
+
+// A chunk: queries [off, off + m) of a request.
+struct SReq {
+  Request* req;
+  uint32_t off, m;
+  double deadline;  // ms since server creation
+  uint64_t seq;     // submission order, ties
+};
+
+struct ChunkLater {
+  bool operator()(const SReq& a, const SReq& b) const {
+    return a.deadline != b.deadline ? a.deadline > b.deadline : a.seq > b.seq;
+  }
 };
 
 struct BatchSlot {
@@ -159,7 +181,9 @@ struct mlod_server {
   int device = 0;
   std::vector<mlod_tree*> trees;
   std::vector<mlod::BatchSlot> slots;
-  std::deque<mlod::SReq> pending;
+  uint32_t chunk = 0;
+  std::priority_queue<mlod::SReq, std::vector<mlod::SReq>, mlod::ChunkLater> pending;
+  uint64_t nextSeq = 0;
   std::vector<mlod_server_done> done;
   std::mutex mu;
   std::condition_variable cv, cvDone;
@@ -216,14 +240,14 @@ hipError_t alloc_slot(BatchSlot& b, uint32_t cap) {
   return hipMalloc(&b.sortTmp, b.sortBytes + 16);
 }
 
-// Takes pending requests, oldest first, that fit one batch.  Caller holds mu.
+// Takes pending chunks, earliest deadline first, that fit one batch.  Caller holds mu.
 void take_requests(mlod_server* S, BatchSlot& b) {
   b.reqs.clear();
   uint32_t slots = 0;
-  while (!S->pending.empty() && b.reqs.size() < kMaxReqPerBatch && slots + S->pending.front().m <= S->cap) {
-    slots += S->pending.front().m;
-    b.reqs.push_back(S->pending.front());
-    S->pending.pop_front();
+  while (!S->pending.empty() && b.reqs.size() < kMaxReqPerBatch && slots + S->pending.top().m <= S->cap) {
+    slots += S->pending.top().m;
+    b.reqs.push_back(S->pending.top());
+    S->pending.pop();
   }
 }
 
@@ -231,11 +255,12 @@ void take_requests(mlod_server* S, BatchSlot& b) {
 int launch_batch(mlod_server* S, BatchSlot& b) {
   // group by kernel type: each type is one contiguous run of workgroups and one launch
   std::stable_sort(b.reqs.begin(), b.reqs.end(),
-                   [](const SReq& x, const SReq& y) { return x.kernel.type < y.kernel.type; });
+                   [](const SReq& x, const SReq& y) { return x.req->kernel.type < y.req->kernel.type; });
   uint32_t off = 0, blk = 0, typeBeg[3] = {0, 0, 0}, typeEnd[3] = {0, 0, 0};
   bool seen[3] = {false, false, false};
   for (size_t r = 0; r < b.reqs.size(); ++r) {
-    const SReq& q = b.reqs[r];
+    const SReq& c = b.reqs[r];
+    const Request& q = *c.req;
     const mlod_tree* t = S->trees[q.tree];
     ReqDesc& d = b.hReq[r];
     d.tv = t->view();
@@ -245,22 +270,22 @@ int launch_batch(mlod_server* S, BatchSlot& b) {
     d.minDepth = q.minDepth;
     d.nIter = q.nIter;
     d.slot0 = off;
-    d.count = q.m;
+    d.count = c.m;
     d.pad0 = blk * kBatchWG;
     for (int a = 0; a < 3; ++a) {
       const float ext = t->BB[a] - t->bb[a];
       d.org[a] = t->bb[a];
       d.scale[a] = ext > 0.f ? 1024.f / ext : 0.f;
     }
-    std::memcpy(b.hIn + 3 * (size_t)off, q.in, (size_t)q.m * 3 * sizeof(float));
-    const uint32_t nb = (q.m + kBatchWG - 1) / kBatchWG;
+    std::memcpy(b.hIn + 3 * (size_t)off, q.in + 3 * (size_t)c.off, (size_t)c.m * 3 * sizeof(float));
+    const uint32_t nb = (c.m + kBatchWG - 1) / kBatchWG;
     for (uint32_t i = 0; i < nb; ++i) b.hBlockReq[blk + i] = (uint32_t)r;
     const int kt = q.kernel.type;
     if (!seen[kt]) {
       typeBeg[kt] = blk;
       seen[kt] = true;
     }
-    off += q.m;
+    off += c.m;
     blk += nb;
     typeEnd[kt] = blk;
   }
@@ -298,26 +323,33 @@ int launch_batch(mlod_server* S, BatchSlot& b) {
   if (!e) e = hipEventRecord(b.ev, b.s);
   if (e) return hip_fail(e, "serving batch", __FILE__, __LINE__);
   b.launched = clk::now();
+  for (const SReq& c : b.reqs)
+    if (c.off == 0) c.req->launched = b.launched;
   return MLOD_OK;
 }
 
 // Copies a finished batch's results to the callers (worker thread, without the mutex) and
-// returns their completion records.
-void complete_batch(mlod_server* S, BatchSlot& b, std::vector<mlod_server_done>& out) {
+// returns the completion records of the requests it finished.
+void complete_batch(mlod_server* S, BatchSlot& b, std::vector<mlod_server_done>& out, std::vector<Request*>& finished) {
   for (size_t r = 0; r < b.reqs.size(); ++r) {
-    const SReq& q = b.reqs[r];
+    const SReq& c = b.reqs[r];
+    const Request& q = *c.req;
     const uint32_t off = b.hReq[r].slot0;
-    if (q.outP) std::memcpy(q.outP, b.hOut + 3 * (size_t)off, (size_t)q.m * 3 * sizeof(float));
-    if (q.outN) std::memcpy(q.outN, b.hOut + 3 * ((size_t)S->cap + off), (size_t)q.m * 3 * sizeof(float));
+    if (q.outP) std::memcpy(q.outP + 3 * (size_t)c.off, b.hOut + 3 * (size_t)off, (size_t)c.m * 3 * sizeof(float));
+    if (q.outN)
+      std::memcpy(q.outN + 3 * (size_t)c.off, b.hOut + 3 * ((size_t)S->cap + off), (size_t)c.m * 3 * sizeof(float));
   }
   const auto now = clk::now();
-  for (const SReq& q : b.reqs) {
+  for (const SReq& c : b.reqs) {
+    Request& q = *c.req;
+    if (--q.chunksLeft) continue;
     mlod_server_done d;
     d.ticket = q.ticket;
     d.latency_ms = std::chrono::duration<double, std::milli>(now - q.submitted).count();
-    d.queue_ms = std::chrono::duration<double, std::milli>(b.launched - q.submitted).count();
+    d.queue_ms = std::chrono::duration<double, std::milli>(q.launched - q.submitted).count();
     d.completed_s = std::chrono::duration<double>(now - S->t0).count();
     out.push_back(d);
+    finished.push_back(&q);
   }
 }
 
@@ -333,6 +365,7 @@ void set_error(mlod_server* S, int code, const std::string& msg) {
 void worker_main(mlod_server* S) {
   (void)hipSetDevice(S->device);
   std::vector<mlod_server_done> fin;
+  std::vector<Request*> finished;
   while (true) {
     bool progressed = false, anyBusy = false;
     // completions: copy out, then publish
@@ -341,14 +374,15 @@ void worker_main(mlod_server* S) {
       const hipError_t q = hipEventQuery(b.ev);
       if (q == hipSuccess) {
         fin.clear();
-        complete_batch(S, b, fin);
-        const size_t nr = b.reqs.size();
+        finished.clear();
+        complete_batch(S, b, fin, finished);
         b.reqs.clear();
         b.busy = false;
+        for (Request* q : finished) delete q;
         {
           std::lock_guard<std::mutex> lk(S->mu);
           S->done.insert(S->done.end(), fin.begin(), fin.end());
-          S->requests += nr;
+          S->requests += fin.size();
         }
         S->cvDone.notify_all();
         progressed = true;
@@ -398,16 +432,19 @@ using namespace mlod;
 
 extern "C" {
 
-int mlod_server_create(uint32_t max_batch_queries, uint32_t inflight, int32_t semantics, int32_t precision,
-                       mlod_server** out) {
+int mlod_server_create(uint32_t max_batch_queries, uint32_t chunk_queries, uint32_t inflight, int32_t semantics,
+                       int32_t precision, mlod_server** out) {
   if (!out) return fail(MLOD_ERR_INVALID, "out is NULL");
   *out = nullptr;
-  if (max_batch_queries == 0 || inflight == 0 || inflight > 8) return fail(MLOD_ERR_INVALID, "bad batch capacity or inflight");
+  if (max_batch_queries < kBatchWG || inflight == 0 || inflight > 8)
+    return fail(MLOD_ERR_INVALID, "bad batch capacity or inflight");
+  if (chunk_queries > max_batch_queries) return fail(MLOD_ERR_INVALID, "chunk larger than a batch");
   if (precision != MLOD_PREC_FP32 && precision != MLOD_PREC_FP32C)
     return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
   if (semantics != MLOD_SEM_REF_GPU && semantics != MLOD_SEM_REF_CPU) return fail(MLOD_ERR_INVALID, "bad semantics");
   mlod_server* S = new mlod_server();
   S->cap = max_batch_queries;
+  S->chunk = chunk_queries ? chunk_queries : std::max<uint32_t>(kBatchWG, max_batch_queries / 4 / kBatchWG * kBatchWG);
   S->semantics = semantics;
   S->precision = precision;
   (void)hipGetDevice(&S->device);
@@ -433,6 +470,14 @@ int mlod_server_destroy(mlod_server* S) {
   }
   S->cv.notify_all();
   if (S->worker.joinable()) S->worker.join();
+  // requests left behind by a failed worker
+  std::vector<Request*> left;
+  for (BatchSlot& b : S->slots)
+    for (const SReq& c : b.reqs) left.push_back(c.req);
+  for (; !S->pending.empty(); S->pending.pop()) left.push_back(S->pending.top().req);
+  std::sort(left.begin(), left.end());
+  left.erase(std::unique(left.begin(), left.end()), left.end());
+  for (Request* q : left) delete q;
   for (BatchSlot& b : S->slots) free_slot(b);
   delete S;
   return MLOD_OK;
@@ -454,30 +499,38 @@ int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, f
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->error) return fail(S->error, S->errmsg);
   }
-  if (m == 0 || m > S->cap) return fail(MLOD_ERR_INVALID, "request size must be in 1..max_batch_queries");
+  if (m == 0) return fail(MLOD_ERR_INVALID, "empty request");
   if (k->type < 0 || k->type > 2) return fail(MLOD_ERR_INVALID, "bad kernel type");
   if (k->type == MLOD_KERNEL_GAUSSIAN_MULTIPLE && (k->n_sigmas < 1 || k->n_sigmas > kServeMaxSigmas))
     return fail(MLOD_ERR_UNSUPPORTED, "serving takes 1..8 sigmas");
-  SReq r;
-  r.tree = tree_id;
-  r.kernel = *k;
-  r.cutoff = cutoff;
-  r.minDepth = min_depth;
-  r.nIter = n_iter;
-  r.stepMax = step_max;
-  r.in = queries;
-  r.outP = out_pts;
-  r.outN = out_nmls;
-  r.m = m;
+  Request* r = new Request();
+  r->tree = tree_id;
+  r->kernel = *k;
+  r->cutoff = cutoff;
+  r->minDepth = min_depth;
+  r->nIter = n_iter;
+  r->stepMax = step_max;
+  r->in = queries;
+  r->outP = out_pts;
+  r->outN = out_nmls;
+  r->m = m;
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    if (tree_id >= S->trees.size()) return fail(MLOD_ERR_INVALID, "unknown tree id");
-    r.ticket = S->nextTicket++;
-    r.submitted = clk::now();
-    S->pending.push_back(r);
+    if (tree_id >= S->trees.size()) {
+      delete r;
+      return fail(MLOD_ERR_INVALID, "unknown tree id");
+    }
+    r->ticket = S->nextTicket++;
+    r->submitted = clk::now();
+    r->launched = r->submitted;
+    const double now_ms = std::chrono::duration<double, std::milli>(r->submitted - S->t0).count();
+    const double deadline = now_ms + (double)m * std::max<uint32_t>(n_iter, 1u) / kNominalPtsPerMs;
+    r->chunksLeft = (m + S->chunk - 1) / S->chunk;
+    for (uint32_t off = 0; off < m; off += S->chunk)
+      S->pending.push(SReq{r, off, std::min(S->chunk, m - off), deadline, S->nextSeq++});
+    *ticket = r->ticket;
   }
   S->cv.notify_one();
-  *ticket = r.ticket;
   return MLOD_OK;
 }
 

@@ -174,4 +174,33 @@ def make_workload(name: str, scale: float = 1.0):
         sig = [0.01 * 2.0 ** i * R for i in range(6)]
         return dict(points=p, normals=nr, queries=q, kernel=dict(type=2, sigmas=sig), cutoff=2.0,
                     description=f"superquadric N=M={n}, multi-Gaussian S=6, cutoff 2, offsets +-0.02 diag")
+    if name in ("c5_knn", "c5_ball"):
+        # global-APSS variants at scale: 20M points on the displaced superquadric with 10 holes;
+        # queries = centres of the depth-11 cells of [-1, 1]^3 enclosing the points
+        # (generateFromPoints, upsample.cpp:128-144), capped at 16M
+        n = max(64, int(20_000_000 * scale))
+        p, nr = superquadric(n, seed=5, holes=10)
+        q = cell_centres_of_points(p, 11, cap=max(64, int(16_000_000 * scale)), seed=55)
+        R = bbox_radius(p)
+        sig = [0.01 * 2.0 ** i * R for i in range(4)]
+        mode = "knn" if name == "c5_knn" else "ball"
+        return dict(points=p, normals=nr, queries=q, kernel=dict(type=2, sigmas=sig), cutoff=2.0, mode=mode, knn=32,
+                    ball_radius=-1.0,
+                    description=f"superquadric N={n} (10 holes), {len(q)} depth-11 cell centres, "
+                                f"{'kNN k=32' if mode == 'knn' else 'ball r=3 sigma0'}, multi-Gaussian S=4")
     raise KeyError(name)
+
+
+def cell_centres_of_points(p: np.ndarray, depth: int, cap: int, seed: int) -> np.ndarray:
+    """Centres of the cells of the [-1, 1]^3 octree at ``depth`` that contain a point of ``p``
+    (float32 arithmetic of aabb_octree::enclosing / center), at most ``cap`` of them (a seeded
+    random subset, kept in cell order)."""
+    t = ((p.astype(np.float32) - np.float32(-1.0)) / np.float32(2.0)) / np.float32(2.0 ** -depth)
+    c = np.floor(t).astype(np.int64)
+    key = (c[:, 2] << 42) | ((c[:, 1] & 0x1fffff) << 21) | (c[:, 0] & 0x1fffff)
+    _, first = np.unique(key, return_index=True)
+    cells = c[np.sort(first)]
+    if len(cells) > cap:
+        keep = np.sort(np.random.default_rng(seed).choice(len(cells), cap, replace=False))
+        cells = cells[keep]
+    return ((2 * cells + 1).astype(np.float32) * np.float32(2.0 ** -depth) - np.float32(1.0)).astype(np.float32)

@@ -30,13 +30,14 @@ __all__ = ["Server", "serving_workload", "run_load"]
 
 class Server:
     """A serving instance: resident trees, a worker thread, up to ``inflight`` batches of at
-    most ``max_batch`` queries on the GPU at once."""
+    most ``max_batch`` queries on the GPU at once; requests are cut into chunks of ``chunk``
+    queries (0: max_batch / 4)."""
 
-    def __init__(self, max_batch=262144, inflight=3, semantics="ref_gpu", precision="fp32c"):
+    def __init__(self, max_batch=262144, inflight=4, semantics="ref_gpu", precision="fp32c", chunk=0):
         sem = {"ref_gpu": 0, "ref_cpu": 1}[semantics]
         prec = {"fp32": 0, "fp32c": 1, "fp64": 2}[precision]
         h = C.c_void_p()
-        check(lib().mlod_server_create(int(max_batch), int(inflight), sem, prec, C.byref(h)))
+        check(lib().mlod_server_create(int(max_batch), int(chunk), int(inflight), sem, prec, C.byref(h)))
         self.h = h
         self.max_batch = int(max_batch)
         self._keep = {}
@@ -203,8 +204,9 @@ def main():
     ap.add_argument("--rate", type=float, nargs="+", default=[2000.0])
     ap.add_argument("--seconds", type=float, default=2.0)
     ap.add_argument("--clouds", type=int, default=64)
-    ap.add_argument("--max-batch", type=int, default=524288)
-    ap.add_argument("--inflight", type=int, default=3)
+    ap.add_argument("--max-batch", type=int, default=262144)
+    ap.add_argument("--chunk", type=int, default=65536)
+    ap.add_argument("--inflight", type=int, default=4)
     ap.add_argument("--max-pts", type=int, default=4_000_000)
     args = ap.parse_args()
     log = lambda *a: print("[serve]", *a, file=sys.stderr, flush=True)  # noqa: E731
@@ -214,7 +216,7 @@ def main():
     t0 = time.time()
     trees = [APSSOctree().build(p, n) for p, n in clouds]
     log(f"octrees built in {time.time() - t0:.1f}s, {sum(t.info()['device_bytes'] for t in trees) / 2**30:.1f} GiB resident")
-    srv = Server(max_batch=args.max_batch, inflight=args.inflight)
+    srv = Server(max_batch=args.max_batch, inflight=args.inflight, chunk=args.chunk)
     out = []
     for r in args.rate:
         res, _, _ = run_load(srv, trees, clouds, rate=r, seconds=args.seconds, log=log)

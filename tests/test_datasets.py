@@ -28,3 +28,18 @@ def test_workloads_small():
     assert len(w["points"]) == 1000 and w["kernel"]["type"] == 2 and len(w["kernel"]["sigmas"]) == 4
     w = ds.make_workload("torus1m", scale=0.001)
     assert w["kernel"]["type"] == 1 and w["queries"].shape == w["points"].shape
+
+
+def test_cell_centres_of_points_match_generator_oracle(oracle_built):
+    """The C5 query set (centres of the depth-d cells enclosing the points) equals the oracle's
+    generateFromPoints + aabb_octree::center."""
+    p, _ = ds.superquadric(20000, seed=9, holes=3)
+    q = ds.cell_centres_of_points(p, 7, cap=10 ** 9, seed=1)
+    og = oracle_built.OracleGenerator(None)
+    cells = og.generate_from_points(p, 7)
+    oc = np.array([oracle_built.oracle_cell_center(c) for c in cells])
+    key = lambda a: np.lexsort((a[:, 0], a[:, 1], a[:, 2]))  # noqa: E731
+    assert len(q) == len(oc)
+    assert np.array_equal(q[key(q)], oc[key(oc)])
+    capped = ds.cell_centres_of_points(p, 7, cap=100, seed=1)
+    assert len(capped) == 100 and len(np.unique(capped, axis=0)) == 100

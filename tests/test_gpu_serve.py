@@ -68,13 +68,14 @@ def test_alone_equals_mixed_equals_standalone(trees, semantics, precision):
     alone = _serve(srv_a, tids, reqs, together=False)
     assert srv_a.stats()["batches"] == len(reqs)
     srv_a.close()
-    srv_b = Server(max_batch=8192, inflight=3, semantics=semantics, precision=precision)
+    srv_b = Server(max_batch=8192, inflight=3, semantics=semantics, precision=precision, chunk=1024)
     tids = [srv_b.add_tree(t) for t in tr]
     mixed = _serve(srv_b, tids, reqs, together=True)
     st = srv_b.stats()
     srv_b.close()
     assert st["requests"] == len(reqs) and st["queries"] == sum(len(r["q"]) for r in reqs)
     assert st["batches"] < len(reqs)  # requests were packed together
+    assert max(len(r["q"]) for r in reqs) > 1024  # and some were cut into chunks
     for r, (pa, na), (pm, nm) in zip(reqs, alone, mixed):
         assert np.array_equal(pa, pm, equal_nan=True) and np.array_equal(na, nm, equal_nan=True)
         ps, ns, _ = gpu_project(tr[r["tree"]], r["q"], r["kernel"], cutoff=r["cutoff"], min_depth=r["min_depth"],
@@ -85,13 +86,13 @@ def test_alone_equals_mixed_equals_standalone(trees, semantics, precision):
 
 def test_server_rejects_bad_requests(trees):
     clouds, tr = trees
-    srv = Server(max_batch=1000, inflight=1)
+    srv = Server(max_batch=1024, inflight=1)
     tid = srv.add_tree(tr[0])
     q = np.zeros((1001, 3), np.float32)
     k = Kernel(type=0, sigma=0.1)
     from mlodsurfaces_amd._lib import MlodError
     with pytest.raises(MlodError):
-        srv.submit(tid, q, k)  # larger than a batch
+        srv.submit(tid, q[:0], k)  # empty
     with pytest.raises(MlodError):
         srv.submit(tid + 5, q[:10], k)  # unknown tree
     with pytest.raises(MlodError):
@@ -99,3 +100,5 @@ def test_server_rejects_bad_requests(trees):
     srv.close()
     with pytest.raises(MlodError):
         Server(precision="fp64")
+    with pytest.raises(MlodError):
+        Server(max_batch=4096, chunk=8192)  # chunk larger than a batch
