@@ -101,6 +101,37 @@ def cpu_baseline(w, kern, cutoff, budget_s=15.0):
                       f"{build_s:.1f} s untimed"}
 
 
+def adaptive_measure(A, ds, w, depth, semantics, precision):
+    """OctreeGenerator::generate at `depth` on the workload's cloud in the reference's normalised
+    frame (Pn::normalize), projecting with the workload's kernel: wall time, cells, projections."""
+    from mlodsurfaces_amd import hiprt
+    p = ds.pn_normalize(w["points"])
+    t = A.APSSOctree().build(p, w["normals"])
+    k = A.Kernel(**w["kernel"])
+    ap = A.Fastapss(t, k)
+    ap.setScalingProtectionSphere(w["cutoff"])
+    ap.set_semantics(semantics)
+    ap.set_precision(precision)
+    gen = A.OctreeGenerator(ap, k)
+    gen.generate(max(1, depth - 2))  # warm-up
+    hiprt.device_synchronize()
+    t0 = time.perf_counter()
+    total = []
+    leaves = gen.generate(depth, total)
+    dt = time.perf_counter() - t0
+    inf = gen.info()
+    log(f"adaptive octree depth {depth}: {len(leaves)} cells, {len(total[0])} crossed, {inf['calls']} projection "
+        f"calls of {inf['projected']} centres, {dt * 1e3:.1f} ms")
+    return {"max_depth": depth, "max_relative_depth": inf["max_rel_depth"], "cells": len(leaves),
+            "crossed_cells": len(total[0]), "projected_centres": inf["projected"], "projection_calls": inf["calls"],
+            "ms": dt * 1e3}
+
+
+def lib_handle():
+    from mlodsurfaces_amd import _lib
+    return _lib.lib()
+
+
 def calibrate_copy():
     """A kernel of known traffic for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: out = 2 * in over
     128 Mi floats, i.e. 512 MiB read and 512 MiB written per dispatch, three dispatches."""
@@ -171,6 +202,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--serve-clouds", type=int, default=64)
+    ap.add_argument("--adaptive-depth", type=int, default=10,
+                    help="max depth of the adaptive octree timed with the torus1m workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain elementwise copy of 512 MiB three times (to scale the HBM counters)")
     args = ap.parse_args()
@@ -246,7 +279,9 @@ def main():
         launch_ms = [ev[s].elapsed_ms(ev[s + 1]) for s in range(args.steps)]
         total_ms = ev[0].elapsed_ms(ev[-1])
         pts, nrm = ap_.download(m, stream=stream.ptr)
-        results[form] = dict(total_ms=total_ms, wall_s=wall, launch_ms=launch_ms, pts=pts, nrm=nrm,
+        kinds = ap_.fit_kinds(m, stream=stream.ptr) if mode in ("mlod", "global") and \
+            hasattr(lib_handle(), "mlod_queries_fit_kinds") else None
+        results[form] = dict(total_ms=total_ms, wall_s=wall, launch_ms=launch_ms, pts=pts, nrm=nrm, kinds=kinds,
                              pts_per_s=m * args.steps / (total_ms / 1e3))
         log(f"{form}: {args.steps} steps in {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms), "
             f"{results[form]['pts_per_s'] / 1e6:.1f} M projected pts/s, mean launch {np.mean(launch_ms):.2f} ms")
@@ -258,6 +293,10 @@ def main():
             e = np.linalg.norm(r["pts"].astype(np.float64) - ref64["pts"].astype(np.float64), axis=1) / diag
             ok = np.isfinite(e)
             errs[form] = dict(max=float(e[ok].max()), rms=float(np.sqrt(np.mean(e[ok] ** 2))), nonfinite=int((~ok).sum()))
+            if r["kinds"] is not None and ref64["kinds"] is not None:
+                # plane / sphere switch flips of the last step against the FP64 form
+                errs[form]["plane_sphere_flips"] = int(np.sum((r["kinds"] == 1) != (ref64["kinds"] == 1)))
+                errs[form]["plane_fits"] = int(np.sum(r["kinds"] == 1))
             log(f"{form} vs fp64 after {args.steps} steps: max {errs[form]['max']:.3e} rms {errs[form]['rms']:.3e} diag")
 
     head = results[args.precision]
@@ -280,6 +319,9 @@ def main():
     if args.workload == "superquadric8m" and args.scale == 1.0 and args.precision == "fp32c" and \
             args.semantics == "ref_gpu":
         roofline["traffic"] = traffic_from_profile("k_project", args.workload)
+    adaptive = None
+    if args.workload == "torus1m":
+        adaptive = adaptive_measure(A, ds, w, args.adaptive_depth, args.semantics, args.precision)
     del tree, dP, dN, dQ
     hiprt.device_synchronize()
     serving = None
@@ -322,6 +364,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "adaptive_octree": adaptive,
     }
     print(json.dumps(out), flush=True)
 

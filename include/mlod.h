@@ -121,6 +121,12 @@ int mlod_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, floa
 /* Copies the current positions / normals of the first n queries to host arrays. */
 int mlod_queries_download(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, void* stream);
 
+/* Which branch of the sphere fit (Fastapss.cu:354-395 / CPUUtils.cpp:739-782) the last MLoD or
+ * global projection step of each of the first n queries took: 0 sphere, 1 plane (|u4| below the
+ * semantics' threshold), 2 invalid fit (query kept, zero normal).  Diagnostic, for counting
+ * plane/sphere switches between arithmetic forms. */
+int mlod_queries_fit_kinds(mlod_tree* t, uint32_t n, uint8_t* out, void* stream);
+
 /* APSS::projectCPU (GPUUtils.h:33-34; APSSOctree::projectCPU, CPUUtils.cpp:621-636), batched:
  * n_iter steps of m host points (m x 3) with the tree's current settings, results to out_pts /
  * out_nmls (host, m x 3).  The uploaded queries are left untouched.  The reference's CPU function

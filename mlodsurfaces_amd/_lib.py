@@ -74,6 +74,7 @@ SIGNATURES = {
     "mlod_server_stats": (C.c_int, [_vp, _u64]),
     "mlod_project_points": (C.c_int, [_vp, _f, C.c_uint32, C.c_uint32, C.c_float, _f, _f, _vp]),
     "mlod_queries_reorganize": (C.c_int, [_vp, _u, C.c_uint32, C.c_uint32, _vp]),
+    "mlod_queries_fit_kinds": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
     "mlod_generator_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "mlod_generator_destroy": (C.c_int, [_vp]),
     "mlod_generate": (C.c_int, [_vp, C.c_uint32, C.c_int32, _vp]),
@@ -99,7 +100,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m mlodsurfaces_amd.build`")
         L = C.CDLL(LIB_PATH)
+        override = bool(os.environ.get("MLOD_LIB_PATH"))
         for name, (res, args) in SIGNATURES.items():
+            if override and not hasattr(L, name):
+                continue  # an older library given explicitly: its missing entries fail when called
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args

@@ -358,6 +358,13 @@ class APSS:
         check(lib().mlod_queries_reorganize(self.m_apssOctree.handle, inv.ctypes.data_as(C.POINTER(C.c_uint32)),
                                             len(inv), int(endValid), stream))
 
+    def fit_kinds(self, n=None, stream=None):
+        """Branch of the last step's sphere fit per query: 0 sphere, 1 plane, 2 invalid."""
+        n = self._m if n is None else int(n)
+        out = np.zeros(n, np.uint8)
+        check(lib().mlod_queries_fit_kinds(self.m_apssOctree.handle, n, out.ctypes.data_as(C.c_void_p), stream))
+        return out
+
     def download(self, n=None, stream=None):
         n = self._m if n is None else int(n)
         op = np.zeros((n, 3), np.float32)

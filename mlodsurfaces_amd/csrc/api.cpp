@@ -227,6 +227,11 @@ int mlod_project_points(mlod_tree* t, const float* xyz, uint32_t m, uint32_t n_i
   return r;
 }
 
+int mlod_queries_fit_kinds(mlod_tree* t, uint32_t n, uint8_t* out, void* stream) {
+  if (!t || !out) return fail(MLOD_ERR_INVALID, "NULL argument");
+  return queries_fit_kinds(t, n, out, (hipStream_t)stream);
+}
+
 int mlod_queries_reorganize(mlod_tree* t, const uint32_t* invalid, uint32_t n, uint32_t end_valid, void* stream) {
   if (!t) return fail(MLOD_ERR_INVALID, "tree is NULL");
   if (n && !invalid) return fail(MLOD_ERR_INVALID, "invalid list is NULL");

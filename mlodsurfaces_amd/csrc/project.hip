@@ -477,8 +477,10 @@ __device__ __forceinline__ T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b
 // u0 + tau |u| + u4 tau^2 = 0 along the line through q and the centre.  It is the same point
 // (|c| - r = 2 u0 / (|u| + sqrt(|u|^2 - 4 u4 u0)) up to sign) without the cancellation of c and
 // r near planar fits (|u4| -> 0, |c| ~ 1/|u4|), and tends to the plane projection as u4 -> 0.
+// kind: 0 sphere, 1 plane (|u4| below the threshold), 2 invalid (NaN: the query is kept)
 template <class T, class TF, int SEM, bool STABLE = false>
-__device__ __forceinline__ void fit(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T> q, T stepMax, V3<T>& outP, V3<T>& outN) {
+__device__ __forceinline__ void fit(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T> q, T stepMax, V3<T>& outP, V3<T>& outN,
+                                    float& kind) {
   const T A = spn - dot3(sp, V3<T>{sn.x / sw, sn.y / sw, sn.z / sw});
   const T B = spp - dot3(sp, V3<T>{sp.x / sw, sp.y / sw, sp.z / sw});
   const T u4 = (T)((0.5 * (double)A) / (double)B);
@@ -487,6 +489,7 @@ __device__ __forceinline__ void fit(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T
   const T u0 = -(dot3(sp, u) + u4 * spp) / sw;
   V3<T> p = q, nr;
   const double tau = SEM == MLOD_SEM_REF_GPU ? 0.000001 : 0.000000000001;
+  kind = fabs((double)u4) < tau ? 1.f : 0.f;
   if (fabs((double)u4) < tau) {
     const V3<T> nn{-u.x, -u.y, -u.z};
     const T lambda = (u0 - dot3(p, nn)) / dot3(nn, nn);
@@ -516,6 +519,7 @@ __device__ __forceinline__ void fit(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T
   if (isnan(p.x) || isnan(p.y) || isnan(p.z)) {
     outP = q;
     outN = V3<T>{0, 0, 0};
+    kind = 2.f;
     return;
   }
   const V3<T> dlt{p.x - q.x, p.y - q.y, p.z - q.z};
@@ -534,24 +538,27 @@ __device__ __forceinline__ void fit_step(const typename FormT<FORM>::Acc& acc, V
                                          float4& nout) {
   if constexpr (FORM == 0) {
     V3<float> p, n;
+    float kind;
     fit<float, float, SEM>(acc.w, V3<float>{acc.px, acc.py, acc.pz}, V3<float>{acc.nx, acc.ny, acc.nz}, acc.pn, acc.pp, qf,
-                           stepMax, p, n);
+                           stepMax, p, n, kind);
     qd = V3<double>{p.x, p.y, p.z};
-    nout = make_float4(n.x, n.y, n.z, 0.f);
+    nout = make_float4(n.x, n.y, n.z, kind);
   } else if constexpr (FORM == 1) {
     // query frame: the query is the origin
     V3<double> p, n;
+    float kind;
     fit<double, double, SEM, true>(acc.w.val(), V3<double>{acc.px.val(), acc.py.val(), acc.pz.val()},
                              V3<double>{acc.nx.val(), acc.ny.val(), acc.nz.val()}, acc.pn.val(), acc.pp.val(),
-                             V3<double>{0, 0, 0}, (double)stepMax, p, n);
+                             V3<double>{0, 0, 0}, (double)stepMax, p, n, kind);
     qd = V3<double>{(double)qf.x + p.x, (double)qf.y + p.y, (double)qf.z + p.z};
-    nout = make_float4((float)n.x, (float)n.y, (float)n.z, 0.f);
+    nout = make_float4((float)n.x, (float)n.y, (float)n.z, kind);
   } else {
     V3<double> p, n;
+    float kind;
     fit<double, double, SEM>(acc.w, V3<double>{acc.px, acc.py, acc.pz}, V3<double>{acc.nx, acc.ny, acc.nz}, acc.pn, acc.pp,
-                             qd, (double)stepMax, p, n);
+                             qd, (double)stepMax, p, n, kind);
     qd = p;
-    nout = make_float4((float)n.x, (float)n.y, (float)n.z, 0.f);
+    nout = make_float4((float)n.x, (float)n.y, (float)n.z, kind);
   }
 }
 

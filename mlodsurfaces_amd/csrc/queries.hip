@@ -68,6 +68,15 @@ __global__ __launch_bounds__(BS) void k_scatter(const float4* __restrict__ src, 
   out[3 * (uint64_t)d + 2] = v.z;
 }
 
+// the fit kind of the last step (stored in the normal's w) in the caller's order
+__global__ __launch_bounds__(BS) void k_scatter_kind(const float4* __restrict__ qn, const uint32_t* __restrict__ perm,
+                                                     uint32_t m, uint32_t limit, uint8_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t d = perm[i];
+  if (d < limit) out[d] = (uint8_t)qn[i].w;
+}
+
 // inverse of the sort permutation: caller index -> sorted slot
 __global__ __launch_bounds__(BS) void k_invperm(const uint32_t* __restrict__ perm, uint32_t m, uint32_t* __restrict__ inv) {
   const uint32_t j = blockIdx.x * BS + threadIdx.x;
@@ -178,6 +187,21 @@ int queries_get(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, hipSt
 }  // namespace mlod
 
 namespace mlod {
+
+int queries_fit_kinds(mlod_tree* t, uint32_t n, uint8_t* out, hipStream_t s) {
+  QueryState& qs = t->qs;
+  if (n > qs.m) return fail(MLOD_ERR_INVALID, "n exceeds the number of uploaded queries");
+  if (n == 0) return MLOD_OK;
+  uint8_t* d = nullptr;
+  if (hipMalloc((void**)&d, n) != hipSuccess) return fail(MLOD_ERR_OOM, "fit kind buffer");
+  k_scatter_kind<<<(qs.m + BS - 1) / BS, BS, 0, s>>>(qs.qn, qs.perm, qs.m, n, d);
+  hipError_t e = hipGetLastError();
+  if (!e) e = hipMemcpyAsync(out, d, n, hipMemcpyDeviceToHost, s);
+  if (!e) e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  if (e) return hip_fail(e, "fit kinds", __FILE__, __LINE__);
+  return MLOD_OK;
+}
 
 int queries_reorganize(mlod_tree* t, const uint32_t* invalid, uint32_t n, uint32_t end_valid, hipStream_t s) {
   QueryState& qs = t->qs;

@@ -138,6 +138,19 @@ def offset_queries(p: np.ndarray, m: int, amp: float, seed: int):
     return (p[j].astype(np.float64) + rng.uniform(-amp, amp, (m, 3))).astype(np.float32)
 
 
+def pn_normalize(p: np.ndarray) -> np.ndarray:
+    """The reference application's normalisation of a loaded cloud (Pn::normalize,
+    sources/apss/Pn.cpp:262-290): centroid to the origin, then division by the largest
+    |coordinate| of the bounding box grown by 0.5 % of its diagonal, so the cloud sits inside
+    [-1, 1]^3 with a margin (the frame of the adaptive octree)."""
+    p64 = p.astype(np.float64)
+    off = p64.mean(0)
+    mx, mn = p64.max(0) - off, p64.min(0) - off
+    small = np.linalg.norm(mx - mn) * 0.005
+    norme = max(np.abs(mx + small).max(), np.abs(mn - small).max())
+    return ((p64 - off) / norme).astype(np.float32)
+
+
 def bbox_radius(p: np.ndarray) -> float:
     """Half the diagonal of the float AABB (BBOX::radius, sources/apss/CPUUtils.h:84-85)."""
     lo = p.min(0).astype(np.float32)

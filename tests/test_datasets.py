@@ -43,3 +43,16 @@ def test_cell_centres_of_points_match_generator_oracle(oracle_built):
     assert np.array_equal(q[key(q)], oc[key(oc)])
     capped = ds.cell_centres_of_points(p, 7, cap=100, seed=1)
     assert len(capped) == 100 and len(np.unique(capped, axis=0)) == 100
+
+
+def test_pn_normalize_frame():
+    """Pn::normalize (Pn.cpp:262-290): centroid at the origin, largest |coordinate| of the box
+    grown by 0.5 % of its diagonal equal to 1."""
+    p, _ = ds.torus(20000, seed=4)
+    q = ds.pn_normalize(p * 3.0 + 5.0)
+    assert np.allclose(q.astype(np.float64).mean(0), 0, atol=1e-6)
+    ext = np.abs(q).max()
+    lo, hi = q.min(0).astype(np.float64), q.max(0).astype(np.float64)
+    small = np.linalg.norm(hi - lo) * 0.005
+    assert abs(max(np.abs(hi + small).max(), np.abs(lo - small).max()) - 1.0) < 1e-5
+    assert ext < 1.0

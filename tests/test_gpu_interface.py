@@ -79,3 +79,26 @@ def test_reorganize_points_sequential_exchange(setup):
     assert np.array_equal(P2, FP)
     with pytest.raises(A.MlodError):
         ap.reorganizePoints(np.array([m + 3], np.uint32), end_valid)
+
+
+def test_fit_kinds_plane_and_sphere():
+    """The fit-branch diagnostic: noise-free plane data takes the plane branch, a sphere the
+    sphere branch (Fastapss.cu:354-395)."""
+    rng = np.random.default_rng(74)
+    xy = rng.uniform(-1, 1, (20000, 2)).astype(np.float32)
+    P = np.column_stack([xy, np.zeros(len(xy), np.float32)]).astype(np.float32)
+    N = np.tile(np.array([[0, 0, 1]], np.float32), (len(P), 1))
+    S, SN = ds.sphere(20000, seed=75, noise=0.0)
+    for pts, nrm, want in ((P, N, 1), (S, SN, 0)):
+        t = A.APSSOctree().build(pts, nrm)
+        R = t.radius()
+        K = A.Kernel(type=2, sigmas=[float(np.float32(0.02 * 2 ** i) * np.float32(R)) for i in range(3)])
+        ap = A.Fastapss(t, K)
+        ap.set_precision("fp32c")
+        q = pts[:500] * np.float32(0.8) + nrm[:500] * np.float32(0.01)
+        ap.copyPointsToGPU(len(q), q)
+        ap.project(len(q), None, None, 1, K)
+        kinds = ap.fit_kinds()
+        assert np.all(kinds == want), np.bincount(kinds)
+        with pytest.raises(A.MlodError):
+            ap.fit_kinds(len(q) + 1)
