@@ -178,10 +178,10 @@ int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hi
 int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream_t s);
 
 // serving batch (project.hip): the workgroup-aligned slots of workgroups [blk0, blk1) (each
-// workgroup of kBatchWG slots belongs to request blockReq[block]) with kernel type kt
+// workgroup of kBatchWG slots belongs to request blockReq[block]), all kernel types in one launch
 constexpr uint32_t kBatchWG = 128;
 int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_blockReq, float4* d_q, float4* d_qn, uint32_t blk0,
-                         uint32_t blk1, int kt, int semantics, int precision, hipStream_t s);
+                         uint32_t blk1, int semantics, int precision, hipStream_t s);
 WCoef make_wcoef(const mlod_kernel& k);
 
 // neighbors.hip

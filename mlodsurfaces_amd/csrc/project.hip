@@ -41,9 +41,15 @@ constexpr int kLevels = kMaxDepth;  // internal nodes have depth 0..11
 
 // ----------------------------------------------------------------------------- weights
 // FP32 weights (Kernel.cpp:64-83 / GPUUtils.cu:243-257) from the squared distance.
+// KT < 0: the kernel type is read at run time from the (wave-uniform) coefficients, so one
+// launch serves workgroups of different kernel types with a scalar branch per evaluation.
 template <int KT, class KP>
 __device__ __forceinline__ float weight32(const KP& k, float d2) {
-  if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
+  if constexpr (KT < 0) {
+    if (k.type == MLOD_KERNEL_GAUSSIAN_MULTIPLE) return weight32<MLOD_KERNEL_GAUSSIAN_MULTIPLE>(k, d2);
+    if (k.type == MLOD_KERNEL_GAUSSIAN) return weight32<MLOD_KERNEL_GAUSSIAN>(k, d2);
+    return weight32<MLOD_KERNEL_SINGULAR>(k, d2);
+  } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
     float w = 0.f;
     if (k.ns <= 8) {
       // statically indexed, so the coefficients are loaded once into scalar registers
@@ -692,14 +698,8 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
 }
 
 template <int FORM, int SEM>
-void launch_batch_kt(const ReqDesc* r, const uint32_t* br, float4* q, float4* qn, uint32_t b0, uint32_t b1, int kt,
-                     hipStream_t s) {
-  const uint32_t grid = b1 - b0;
-  switch (kt) {
-    case MLOD_KERNEL_GAUSSIAN: k_project_batch<FORM, SEM, MLOD_KERNEL_GAUSSIAN><<<grid, PBS, 0, s>>>(r, br, q, qn, b0); break;
-    case MLOD_KERNEL_SINGULAR: k_project_batch<FORM, SEM, MLOD_KERNEL_SINGULAR><<<grid, PBS, 0, s>>>(r, br, q, qn, b0); break;
-    default: k_project_batch<FORM, SEM, MLOD_KERNEL_GAUSSIAN_MULTIPLE><<<grid, PBS, 0, s>>>(r, br, q, qn, b0); break;
-  }
+void launch_batch_all(const ReqDesc* r, const uint32_t* br, float4* q, float4* qn, uint32_t b0, uint32_t b1, hipStream_t s) {
+  k_project_batch<FORM, SEM, -1><<<b1 - b0, PBS, 0, s>>>(r, br, q, qn, b0);
 }
 
 template <int FORM, int SEM, int KT, bool CNT>
@@ -780,15 +780,15 @@ WCoef make_wcoef(const mlod_kernel& k) {
 }
 
 int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_blockReq, float4* d_q, float4* d_qn, uint32_t blk0,
-                         uint32_t blk1, int kt, int semantics, int precision, hipStream_t s) {
+                         uint32_t blk1, int semantics, int precision, hipStream_t s) {
   if (blk1 <= blk0) return MLOD_OK;
   const bool cpu = semantics == MLOD_SEM_REF_CPU;
   if (precision == MLOD_PREC_FP32C) {
-    if (cpu) launch_batch_kt<1, MLOD_SEM_REF_CPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, kt, s);
-    else launch_batch_kt<1, MLOD_SEM_REF_GPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, kt, s);
+    if (cpu) launch_batch_all<1, MLOD_SEM_REF_CPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, s);
+    else launch_batch_all<1, MLOD_SEM_REF_GPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, s);
   } else if (precision == MLOD_PREC_FP32) {
-    if (cpu) launch_batch_kt<0, MLOD_SEM_REF_CPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, kt, s);
-    else launch_batch_kt<0, MLOD_SEM_REF_GPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, kt, s);
+    if (cpu) launch_batch_all<0, MLOD_SEM_REF_CPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, s);
+    else launch_batch_all<0, MLOD_SEM_REF_GPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, s);
   } else {
     return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
   }

@@ -15,9 +15,9 @@
 // One batch on its stream: pinned host staging -> device copy; per-slot Morton key in the
 // request's tree box with the request index above it; radix sort (so each wavefront walks one
 // request's neighbourhood); gather into a layout where every request starts on a workgroup
-// boundary; one projection launch per kernel type over that type's workgroups
-// (k_project_batch: the workgroup's request descriptor is uniform and read into scalar
-// registers; per-query code and bits are those of a standalone projection); scatter back to
+// boundary; one projection launch over all workgroups (k_project_batch: the workgroup's request
+// descriptor, kernel type included, is uniform and read into scalar registers; per-query code
+// and bits are those of a standalone projection); scatter back to
 // submission order; device -> pinned copy; completion event.  The worker copies each request's
 // results to the caller's buffers when the event has fired and stamps the completion time.
 // Only the request queue and the completion list are shared with callers (under the mutex);
@@ -253,11 +253,7 @@ void take_requests(mlod_server* S, BatchSlot& b) {
 
 // Stages and launches the requests in b.reqs (worker thread, without the mutex).
 int launch_batch(mlod_server* S, BatchSlot& b) {
-  // group by kernel type: each type is one contiguous run of workgroups and one launch
-  std::stable_sort(b.reqs.begin(), b.reqs.end(),
-                   [](const SReq& x, const SReq& y) { return x.req->kernel.type < y.req->kernel.type; });
-  uint32_t off = 0, blk = 0, typeBeg[3] = {0, 0, 0}, typeEnd[3] = {0, 0, 0};
-  bool seen[3] = {false, false, false};
+  uint32_t off = 0, blk = 0;
   for (size_t r = 0; r < b.reqs.size(); ++r) {
     const SReq& c = b.reqs[r];
     const Request& q = *c.req;
@@ -280,14 +276,8 @@ int launch_batch(mlod_server* S, BatchSlot& b) {
     std::memcpy(b.hIn + 3 * (size_t)off, q.in + 3 * (size_t)c.off, (size_t)c.m * 3 * sizeof(float));
     const uint32_t nb = (c.m + kBatchWG - 1) / kBatchWG;
     for (uint32_t i = 0; i < nb; ++i) b.hBlockReq[blk + i] = (uint32_t)r;
-    const int kt = q.kernel.type;
-    if (!seen[kt]) {
-      typeBeg[kt] = blk;
-      seen[kt] = true;
-    }
     off += c.m;
     blk += nb;
-    typeEnd[kt] = blk;
   }
   const uint32_t slots = off;
   b.slots = slots;
@@ -308,12 +298,7 @@ int launch_batch(mlod_server* S, BatchSlot& b) {
     e = hipGetLastError();
   }
   if (e) return hip_fail(e, "serving batch", __FILE__, __LINE__);
-  for (int kt = 0; kt < 3; ++kt) {
-    if (!seen[kt]) continue;
-    const int r = launch_project_batch(b.dReq, b.dBlockReq, b.dQ, b.dQn, typeBeg[kt], typeEnd[kt], kt, S->semantics,
-                                       S->precision, b.s);
-    if (r) return r;
-  }
+  if (int r = launch_project_batch(b.dReq, b.dBlockReq, b.dQ, b.dQn, 0, blk, S->semantics, S->precision, b.s)) return r;
   k_batch_scatter<<<grid, BS, 0, b.s>>>(b.dQ, b.dQn, b.dKey2, b.dPerm, b.dReq, slots, S->cap, b.dOut);
   e = hipGetLastError();
   if (!e) e = hipMemcpyAsync(b.hOut, b.dOut, (size_t)slots * 3 * sizeof(float), hipMemcpyDeviceToHost, b.s);
