@@ -81,8 +81,10 @@ def cpu_baseline(w, kern, cutoff, budget_s=15.0):
         return None
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     t0 = time.time()
+    log(f"cpu baseline: building the reference octree over {len(w['points'])} points")
     tree = bind.RefTree(w["points"], w["normals"])
     build_s = time.time() - t0
+    log(f"cpu baseline: reference octree built in {build_s:.1f} s")
     k = bind.make_kernel(**kern)
     prm = bind.make_params(cutoff=cutoff, semantics="ref_cpu", mode=w.get("mode", "mlod"), knn=w.get("knn", 20),
                            ball_radius=w.get("ball_radius", 0.1))
@@ -91,6 +93,7 @@ def cpu_baseline(w, kern, cutoff, budget_s=15.0):
     t0 = time.time()
     tree.project(q[:n0], k, prm, n_iter=1, nthreads=cores)
     rate = n0 / max(time.time() - t0, 1e-6)
+    log(f"cpu baseline: calibration sample {n0} queries at {rate:.0f} /s")
     ns = int(min(len(q), max(n0, rate * budget_s)))
     t0 = time.time()
     tree.project(q[:ns], k, prm, n_iter=1, nthreads=cores)
@@ -261,9 +264,16 @@ def main():
         ap_.copyPointsToGPU_device(dQ.ptr.value, m, stream=stream.ptr)
         if counts is None and mode == "mlod":
             counts = ap_.count_visits(stream=stream.ptr)
+            counts["lane_efficiency"] = counts["iterations"] / max(counts["lane_slots"], 1)
             log(f"counters (one step): {counts}")
-        for _ in range(args.warmup):
+        slow = False
+        for wi in range(args.warmup):
+            w0 = time.perf_counter()
             ap_.project(m, None, None, 1, K, stream=stream.ptr)
+            stream.synchronize()
+            if time.perf_counter() - w0 > 2.0:
+                slow = True  # long steps: report each one as it ends
+                log(f"{form}: warm-up step {wi + 1} took {time.perf_counter() - w0:.1f} s")
         ap_.copyPointsToGPU_device(dQ.ptr.value, m, stream=stream.ptr)
         stream.synchronize()
         hiprt.device_synchronize()
@@ -273,6 +283,9 @@ def main():
         for s in range(args.steps):
             ap_.project(m, None, None, 1, K, stream=stream.ptr)
             ev[s + 1].record(stream)
+            if slow:
+                ev[s + 1].synchronize()
+                log(f"{form}: step {s + 1} done")
         ev[-1].synchronize()
         hiprt.device_synchronize()
         wall = time.perf_counter() - wall0

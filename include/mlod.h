@@ -146,10 +146,11 @@ int mlod_tree_dump(const mlod_tree* t, int32_t* depth, float* bb6, float* center
                    int32_t* first_child, int32_t* next_brother, int32_t* num_children, uint32_t* leaf_count,
                    uint32_t* leaf_idx, float* areas, float* moments9);
 
-/* Traversal counters of one step over the uploaded queries (untimed): far nodes
- * accepted, leaf points summed, child nodes tested, near (blended) nodes.
- * out4 receives the four totals. */
-int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out4, void* stream);
+/* Traversal counters of one step over the uploaded queries (untimed).  out6 receives: far
+ * nodes accepted, leaf points summed, child nodes tested, near nodes adding their own share,
+ * traversal loop iterations summed over queries, and 64 x the per-wavefront maximum of those
+ * iterations summed over wavefronts (so out6[4] / out6[5] is the SIMD lane efficiency). */
+int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out6, void* stream);
 
 /* ---------------------------------------------------------------------------------------
  * Serving.  No counterpart in the reference (its harness projects one batch at a time with

@@ -373,12 +373,13 @@ class APSS:
         return op, on
 
     def count_visits(self, n=None, stream=None):
-        """Traversal counters of one step (far nodes, leaf points, children tested, near nodes)."""
+        """Traversal counters of one step: far nodes, leaf points, children tested, near nodes'
+        own shares, loop iterations, and 64 x per-wavefront maximum iterations."""
         n = self._m if n is None else int(n)
         self._sync_params()
-        out = (C.c_uint64 * 4)()
+        out = (C.c_uint64 * 6)()
         check(lib().mlod_count_visits(self.m_apssOctree.handle, n, out, stream))
-        return dict(far=out[0], leaf_points=out[1], tested=out[2], near=out[3])
+        return dict(far=out[0], leaf_points=out[1], tested=out[2], near=out[3], iterations=out[4], lane_slots=out[5])
 
 
 class Fastapss(APSS):

@@ -6,6 +6,8 @@
 #include <cstring>
 #include <string>
 
+#include <vector>
+
 #include "mlod_internal.h"
 
 namespace mlod {
@@ -27,7 +29,7 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 static void free_tree(mlod_tree* t) {
   if (!t) return;
   queries_free(t);
-  void* bufs[] = {t->geo, t->hdr, t->momP, t->momC, t->momD, t->AD, t->pts, t->nrm, t->order, t->box, t->area};
+  void* bufs[] = {t->geo, t->hdr, t->momP, t->momC, t->momD, t->AD, t->pts, t->nrm, t->order, t->box, t->area, t->span};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   delete t;
@@ -253,34 +255,43 @@ int mlod_tree_dump(const mlod_tree* t, int32_t* depth, float* bb6, float* center
                    areas, moments9);
 }
 
-int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out4, void* stream) {
-  if (!t || !out4) return fail(MLOD_ERR_INVALID, "NULL argument");
+int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out6, void* stream) {
+  if (!t || !out6) return fail(MLOD_ERR_INVALID, "NULL argument");
   if (!t->kernel_set) return fail(MLOD_ERR_STATE, "no kernel set (mlod_set_kernel)");
   if (n > t->qs.m) return fail(MLOD_ERR_INVALID, "n exceeds the number of uploaded queries");
   hipStream_t s = (hipStream_t)stream;
   const uint32_t m = t->qs.m;
+  const size_t waves = ((size_t)m + 63) / 64;
+  // 5 totals, then one 32-bit per-wavefront maximum of loop iterations per wavefront
+  const size_t cbytes = 5 * sizeof(unsigned long long) + waves * sizeof(uint32_t);
   float4 *bq = nullptr, *bn = nullptr;
   unsigned long long* dc = nullptr;
   if (hipMalloc((void**)&bq, (size_t)m * sizeof(float4) + 16) != hipSuccess ||
       hipMalloc((void**)&bn, (size_t)m * sizeof(float4) + 16) != hipSuccess ||
-      hipMalloc((void**)&dc, 4 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc((void**)&dc, cbytes) != hipSuccess) {
     (void)hipFree(bq), (void)hipFree(bn), (void)hipFree(dc);
     return fail(MLOD_ERR_OOM, "counter pass buffers");
   }
   hipError_t e = hipMemcpyAsync(bq, t->qs.q, (size_t)m * sizeof(float4), hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(bn, t->qs.qn, (size_t)m * sizeof(float4), hipMemcpyDeviceToDevice, s);
-  if (e == hipSuccess) e = hipMemsetAsync(dc, 0, 4 * sizeof(unsigned long long), s);
+  if (e == hipSuccess) e = hipMemsetAsync(dc, 0, cbytes, s);
   int r = e == hipSuccess ? launch_count(t, n, dc, s) : hip_fail(e, "counter pass setup", __FILE__, __LINE__);
-  unsigned long long h[4] = {0, 0, 0, 0};
+  unsigned long long h[5] = {0, 0, 0, 0, 0};
+  std::vector<uint32_t> wmax(waves);
   if (r == MLOD_OK) {
     e = hipMemcpyAsync(h, dc, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(wmax.data(), dc + 5, waves * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(t->qs.q, bq, (size_t)m * sizeof(float4), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(t->qs.qn, bn, (size_t)m * sizeof(float4), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) r = hip_fail(e, "counter pass", __FILE__, __LINE__);
   }
   (void)hipFree(bq), (void)hipFree(bn), (void)hipFree(dc);
-  for (int k = 0; k < 4; ++k) out4[k] = h[k];
+  uint64_t laneSlots = 0;
+  for (uint32_t w : wmax) laneSlots += 64ull * w;
+  for (int k = 0; k < 5; ++k) out6[k] = h[k];
+  out6[5] = laneSlots;
   return r;
 }
 

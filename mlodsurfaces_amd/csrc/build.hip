@@ -225,6 +225,7 @@ __global__ __launch_bounds__(BS) void k_emit(const uint64_t* __restrict__ codes,
                                              const float* __restrict__ box6, float4* __restrict__ geo,
                                              uint4* __restrict__ hdr, float* __restrict__ box,
                                              uint32_t* __restrict__ nodeLast, uint32_t* __restrict__ pointLeaf,
+                                             uint32_t* __restrict__ span,
                                              unsigned long long* __restrict__ tiesOut) {
   __shared__ uint32_t wt[kMaxDepth + 1][BS / 64];
   const uint32_t i = blockIdx.x * BS + threadIdx.x;
@@ -280,6 +281,7 @@ __global__ __launch_bounds__(BS) void k_emit(const uint64_t* __restrict__ codes,
       const float r = (float)sqrt((double)d2) * 0.5f;
       geo[node] = make_float4(c[0], c[1], c[2], r);
       hdr[node] = make_uint4(first, ((uint32_t)d << 27) | (leaf ? kLeafBit : 0u), 0u, father);
+      span[2 * (uint64_t)node] = i;  // point i starts the subtrees of the nodes it owns
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         box[6 * (uint64_t)node + a] = lo[a];
@@ -292,6 +294,7 @@ __global__ __launch_bounds__(BS) void k_emit(const uint64_t* __restrict__ codes,
       const int dn = d < kMaxDepth ? d + 1 : kMaxDepth;
       const uint32_t last = leaf ? i : lev.v[d + 1] + rank[dn] + (own[dn] ? 1u : 0u) - 1u;
       nodeLast[node] = last;
+      span[2 * (uint64_t)node + 1] = i + 1u;  // and ends those of depth >= endLo
     }
     if (d == sp.L) pointLeaf[i] = lev.v[d] + rank[d] + (own[d] ? 1u : 0u) - 1u;  // the leaf holding i
     if (d < sp.L) {  // the split below this node: the path code fixes the child
@@ -601,11 +604,12 @@ int build_tree(mlod_tree* t, const float* d_xyz, const float* d_nrm, hipStream_t
       (r = dalloc(&t->momP, 2 * (size_t)nn, &t->bytes)) || (r = dalloc(&t->momC, 2 * (size_t)nn, &t->bytes)) ||
       (r = dalloc(&t->momD, 2 * (size_t)nn, &t->bytes)) || (r = dalloc(&t->AD, nn, &t->bytes)) ||
       (r = dalloc(&t->box, 6 * (size_t)nn, &t->bytes)) || (r = dalloc(&t->pts, n, &t->bytes)) ||
-      (r = dalloc(&t->nrm, n, &t->bytes)) || (r = dalloc(&t->area, n, &t->bytes)))
+      (r = dalloc(&t->nrm, n, &t->bytes)) || (r = dalloc(&t->area, n, &t->bytes)) ||
+      (r = dalloc(&t->span, nn, &t->bytes)))
     return r;
   MLOD_HIP(hipMemsetAsync(dties, 0, sizeof(unsigned long long), s));
   k_emit<<<nb, BS, 0, s>>>(codes2, t->order, d_xyz, n, maxDepth, nb, blockOff, lev, dbox, t->geo, t->hdr, t->box, nodeLast,
-                           pointLeaf, dties);
+                           pointLeaf, reinterpret_cast<uint32_t*>(t->span), dties);
   LAUNCH_CHECK();
   k_finalize<<<(nn + BS - 1) / BS, BS, 0, s>>>(nn, t->hdr, nodeLast);
   LAUNCH_CHECK();

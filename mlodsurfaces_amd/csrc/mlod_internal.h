@@ -43,6 +43,7 @@ struct TreeView {
   const double* AD;
   const float4* pts;
   const float4* nrm;
+  const uint2* span;  // per node: sorted points [x, y) of its subtree (leaf order is point order)
   uint32_t n, nn;
 };
 
@@ -122,7 +123,8 @@ struct mlod_tree {
   float4* pts = nullptr;
   float4* nrm = nullptr;
   uint32_t* order = nullptr;
-  float* box = nullptr;   // per node {bb, BB} (parity dumps)
+  float* box = nullptr;   // per node {bb, BB} (parity dumps, neighbour pruning)
+  uint2* span = nullptr;  // per node [first, end) of its subtree's sorted points
   float* area = nullptr;  // per point area, input order (parity dumps)
   // parameters
   mlod_kernel kernel{};
@@ -136,7 +138,7 @@ struct mlod_tree {
   float ball_radius = 0.1f;
   mlod::QueryState qs;
   mlod::TreeView view() const {
-    return mlod::TreeView{geo, hdr, momP, momC, momD, AD, pts, nrm, n, nn};
+    return mlod::TreeView{geo, hdr, momP, momC, momD, AD, pts, nrm, span, n, nn};
   }
 };
 

@@ -13,9 +13,12 @@
 // float, compared as double), so the neighbour sets are the same; the k nearest are ordered
 // by (distance, point index).
 //
-// One lane per query.  Per depth level the lane keeps the first child and an 8-bit mask of
-// children still to visit in LDS.  kNN keeps its candidates in registers as a sorted array of
-// 64-bit keys (distance bits << 32 | sorted point index).
+// kNN: one lane per query.  Per depth level the lane keeps in LDS the first child and the
+// children still to visit in increasing box distance (so the search reaches the query's own
+// leaves first and the k-th distance bound tightens early); small subtrees are scanned as
+// contiguous point ranges.  Candidates are kept in registers as a sorted array of 64-bit keys
+// (distance bits << 32 | sorted point index).  Ball: one walk per wavefront over the union of
+// its lanes' balls (k_ball_wave).
 #include <cfloat>
 #include <cmath>
 
@@ -60,6 +63,9 @@ __device__ __forceinline__ float box_d2(const float* __restrict__ box, uint32_t 
 // Bound slack: box distances are float-rounded, so a node is kept while its bound is within
 // a relative 1e-5 of the search radius (a superset of the exact test).
 constexpr float kSlack = 1.0001f;
+
+// kNN: subtrees of at most this many points are scanned rather than walked.
+constexpr uint32_t kScanPointsKnn = 8;
 
 // Weighted sums of the neighbours (CPUUtils.cpp:719-731): unit area, raw p and n.
 struct SumF {  // plain FP32
@@ -262,7 +268,8 @@ __global__ __launch_bounds__(NBS) void k_knn(TreeView tv, const float* __restric
                                              const uint32_t* __restrict__ perm, uint32_t m, uint32_t limit, uint32_t nIter,
                                              uint32_t k) {
   __shared__ uint32_t sFirst[kLevels][NBS];
-  __shared__ uint8_t sMask[kLevels][NBS];
+  __shared__ uint32_t sOrder[kLevels][NBS];  // children still to visit, nearest first, 4 bits each
+  __shared__ uint8_t sLeft[kLevels][NBS];    // how many
   const int tid = threadIdx.x;
   const uint32_t i = xcd_block() * NBS + tid;
   if (i >= m) return;
@@ -282,32 +289,55 @@ __global__ __launch_bounds__(NBS) void k_knn(TreeView tv, const float* __restric
     while (true) {
       if (have) {
         const uint4 h = cur == 0 ? h0 : tv.hdr[cur];
-        if (hdr_leaf(h.y)) {
-          for (uint32_t j = h.x; j < h.x + hdr_count(h.y); ++j) {
+        const uint2 sp = tv.span[cur];
+        if (hdr_leaf(h.y) || sp.y - sp.x <= kScanPointsKnn) {
+          // a leaf or a small subtree: every point is a candidate (the k nearest do not depend
+          // on the order they are offered in)
+          for (uint32_t j = sp.x; j < sp.y; ++j) {
             const float d2 = kd_d2(tv.pts[j], qf);
             top.insert(((uint64_t)__float_as_uint(d2) << 32) | (j + 1u));
           }
           const uint64_t kk = top.kth();
           bound = kk == ~0ull ? INFINITY : __uint_as_float((uint32_t)(kk >> 32));
         } else {
-          uint32_t mask = 0;
+          // children whose box reaches the bound, visited nearest box first, so that the
+          // first leaves reached are the query's own and the bound tightens at once
           const uint32_t first = h.x, nc = hdr_count(h.y);
-          for (uint32_t c = 0; c < nc; ++c)
-            if (box_d2(box, first + c, qf) <= bound * kSlack) mask |= 1u << c;
+          float d[8];
+#pragma unroll
+          for (uint32_t c = 0; c < 8; ++c) d[c] = c < nc ? box_d2(box, first + c, qf) : INFINITY;
+          uint32_t order = 0, cnt = 0;
+#pragma unroll
+          for (uint32_t c = 0; c < 8; ++c) {
+            const bool in = c < nc && d[c] <= bound * kSlack;
+            uint32_t rank = 0;
+#pragma unroll
+            for (uint32_t o = 0; o < 8; ++o) {
+              const bool oin = o < nc && d[o] <= bound * kSlack;
+              rank += (oin && (d[o] < d[c] || (d[o] == d[c] && o < c))) ? 1u : 0u;
+            }
+            if (in) {
+              order |= c << (4u * rank);
+              ++cnt;
+            }
+          }
           ++lvl;
           sFirst[lvl][tid] = first;
-          sMask[lvl][tid] = (uint8_t)mask;
+          sOrder[lvl][tid] = order;
+          sLeft[lvl][tid] = (uint8_t)cnt;
         }
         have = false;
       }
       if (lvl < 0) break;
-      const uint32_t mask = sMask[lvl][tid];
-      if (mask == 0) {
+      const uint32_t left = sLeft[lvl][tid];
+      if (left == 0) {
         --lvl;
         continue;
       }
-      sMask[lvl][tid] = (uint8_t)(mask & (mask - 1u));
-      const uint32_t c = sFirst[lvl][tid] + (uint32_t)__builtin_ctz(mask);
+      const uint32_t order = sOrder[lvl][tid];
+      sOrder[lvl][tid] = order >> 4;
+      sLeft[lvl][tid] = (uint8_t)(left - 1u);
+      const uint32_t c = sFirst[lvl][tid] + (order & 15u);
       if (box_d2(box, c, qf) <= bound * kSlack) {  // the bound may have shrunk since the push
         cur = c;
         have = true;
@@ -328,71 +358,130 @@ __global__ __launch_bounds__(NBS) void k_knn(TreeView tv, const float* __restric
   qn[i] = nout;
 }
 
+// Ball APSS, one wavefront at a time.  The 64 lanes' queries are Morton neighbours, so their
+// balls overlap almost entirely: the wavefront walks the octree once over the union of its
+// balls (node boxes against the lanes' query box grown by r), takes the points of every
+// reached subtree that is small enough as a contiguous range, loads each range 64 points at a
+// time with one coalesced load per lane, and broadcasts the points lane to lane; every lane
+// sums the points inside its own ball.  Ranges come in increasing point order, so each lane
+// sums exactly the points inside its ball, in increasing point order (the order of a walk over
+// its own ball alone), whatever the other lanes' queries.
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float bcast(float v, uint32_t lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)lane));
+}
+
+// squared distance between a node's box and the box [lo, hi]
+__device__ __forceinline__ float box_box_d2(const float* __restrict__ box, uint32_t c, V3<float> lo, V3<float> hi) {
+  const float2 a = *reinterpret_cast<const float2*>(box + 6 * (size_t)c);
+  const float2 b = *reinterpret_cast<const float2*>(box + 6 * (size_t)c + 2);
+  const float2 e = *reinterpret_cast<const float2*>(box + 6 * (size_t)c + 4);
+  const float dx = fmaxf(fmaxf(a.x - hi.x, lo.x - b.y), 0.f);
+  const float dy = fmaxf(fmaxf(a.y - hi.y, lo.y - e.x), 0.f);
+  const float dz = fmaxf(fmaxf(b.x - hi.z, lo.z - e.y), 0.f);
+  return (dx * dx + dy * dy) + dz * dz;
+}
+
+constexpr uint32_t kWaveScanPoints = 256;  // subtrees up to this size are taken as one range
+
 template <int FORM, int KT>
-__global__ __launch_bounds__(NBS) void k_ball(TreeView tv, const float* __restrict__ box, KParams kp,
-                                              float4* __restrict__ q, float4* __restrict__ qn,
-                                              const uint32_t* __restrict__ perm, uint32_t m, uint32_t limit,
-                                              uint32_t nIter, float r2f) {
-  __shared__ uint32_t sFirst[kLevels][NBS];
-  __shared__ uint8_t sMask[kLevels][NBS];
-  const int tid = threadIdx.x;
+__global__ __launch_bounds__(NBS) void k_ball_wave(TreeView tv, const float* __restrict__ box, KParams kp,
+                                                   float4* __restrict__ q, float4* __restrict__ qn,
+                                                   const uint32_t* __restrict__ perm, uint32_t m, uint32_t limit,
+                                                   uint32_t nIter, float r2f) {
+  constexpr int kWaves = NBS / 64;
+  __shared__ uint32_t sFirst[kLevels][kWaves];
+  __shared__ uint32_t sMask[kLevels][kWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const uint32_t i = xcd_block() * NBS + tid;
-  if (i >= m) return;
-  if (limit < m && perm[i] >= limit) return;
-  const float4 q4 = q[i];
+  const uint32_t i0 = i - (uint32_t)lane;  // the wavefront's first slot
+  if (i0 >= m) return;                      // whole wavefront past the end
+  const bool mine = i < m && !(limit < m && perm[i] >= limit);
+  // lanes without a query of their own follow the wavefront's first query (never stored)
+  const float4 q4 = q[i < m ? i : i0];
   V3<double> qd{q4.x, q4.y, q4.z};
   float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
+  bool done = !mine;
+  const float r = sqrtf(r2f) * kSlack;
   for (uint32_t it = 0; it < nIter; ++it) {
     const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
+    // the union of the live lanes' balls, bounded by their query box grown by r
+    const float big = 3.0e38f;
+    const V3<float> qlo{wave_min(done ? big : qf.x) - r, wave_min(done ? big : qf.y) - r, wave_min(done ? big : qf.z) - r};
+    const V3<float> qhi{wave_max(done ? -big : qf.x) + r, wave_max(done ? -big : qf.y) + r, wave_max(done ? -big : qf.z) + r};
+    if (qlo.x > qhi.x) break;  // every lane finished (uniform)
     typename Sums<FORM>::S s{};
     uint32_t cnt = 0;
-    // walk: leaves sum their points inside the ball, internal nodes push children whose box
-    // reaches the ball
-    const uint4 h0 = tv.hdr[0];
     int lvl = -1;
     uint32_t cur = 0;
     bool have = true;
     while (true) {
       if (have) {
-        const uint4 h = cur == 0 ? h0 : tv.hdr[cur];
-        if (hdr_leaf(h.y)) {
-          for (uint32_t j = h.x; j < h.x + hdr_count(h.y); ++j) {
-            const float4 P = tv.pts[j];
-            if (kd_d2(P, qf) <= r2f) {
-              add_nb<FORM, KT>(s, kp, P, tv.nrm[j], qd, qf);
-              ++cnt;
+        const uint4 h = tv.hdr[cur];
+        const uint2 sp = tv.span[cur];
+        if (hdr_leaf(h.y) || sp.y - sp.x <= kWaveScanPoints) {
+          // the range's points, 64 at a time: one coalesced load per lane, then each point
+          // is broadcast and tested by every lane
+          for (uint32_t base = sp.x; base < sp.y; base += 64) {
+            const uint32_t nb = min(64u, sp.y - base);
+            const uint32_t j = base + (uint32_t)lane;
+            const float4 P = (uint32_t)lane < nb ? tv.pts[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 N = (uint32_t)lane < nb ? tv.nrm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (uint32_t t = 0; t < nb; ++t) {
+              const float4 Pt = make_float4(bcast(P.x, t), bcast(P.y, t), bcast(P.z, t), bcast(P.w, t));
+              if (!done && kd_d2(Pt, qf) <= r2f) {
+                const float4 Nt = make_float4(bcast(N.x, t), bcast(N.y, t), bcast(N.z, t), 0.f);
+                add_nb<FORM, KT>(s, kp, Pt, Nt, qd, qf);
+                ++cnt;
+              }
             }
           }
         } else {
+          const uint32_t first = __builtin_amdgcn_readfirstlane(h.x), nc = hdr_count(h.y);
           uint32_t mask = 0;
-          const uint32_t first = h.x, nc = hdr_count(h.y);
           for (uint32_t c = 0; c < nc; ++c)
-            if (box_d2(box, first + c, qf) <= r2f * kSlack) mask |= 1u << c;
+            if (box_box_d2(box, first + c, qlo, qhi) <= 0.f) mask |= 1u << c;
           ++lvl;
-          sFirst[lvl][tid] = first;
-          sMask[lvl][tid] = (uint8_t)mask;
+          if (lane == 0) {
+            sFirst[lvl][wv] = first;
+            sMask[lvl][wv] = mask;
+          }
         }
         have = false;
       }
       if (lvl < 0) break;
-      const uint32_t mask = sMask[lvl][tid];
+      const uint32_t mask = __builtin_amdgcn_readfirstlane(sMask[lvl][wv]);
       if (mask == 0) {
         --lvl;
         continue;
       }
-      sMask[lvl][tid] = (uint8_t)(mask & (mask - 1u));
-      cur = sFirst[lvl][tid] + (uint32_t)__builtin_ctz(mask);
+      if (lane == 0) sMask[lvl][wv] = mask & (mask - 1u);
+      cur = __builtin_amdgcn_readfirstlane(sFirst[lvl][wv]) + (uint32_t)__builtin_ctz(mask);
       have = true;
     }
-    if (cnt == 0) {  // empty ball: the starting query, zero normal (CPUUtils.cpp:711-716)
-      qd = V3<double>{q4.x, q4.y, q4.z};
-      nout = make_float4(0.f, 0.f, 0.f, 0.f);
-      break;
+    if (!done) {
+      if (cnt == 0) {  // empty ball: the starting query, zero normal (CPUUtils.cpp:711-716)
+        qd = V3<double>{q4.x, q4.y, q4.z};
+        nout = make_float4(0.f, 0.f, 0.f, 0.f);
+        done = true;
+      } else {
+        fit_step<FORM>(s, qd, qf, nout);
+      }
     }
-    fit_step<FORM>(s, qd, qf, nout);
   }
-  q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
-  qn[i] = nout;
+  if (mine) {
+    q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
+    qn[i] = nout;
+  }
 }
 
 template <int FORM, int KT>
@@ -423,7 +512,7 @@ int launch_form(mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter,
     const double r2 = (double)r * (double)r;
     float r2f = (float)r2;
     if ((double)r2f > r2) r2f = nextafterf(r2f, 0.f);
-    k_ball<FORM, KT><<<grid, NBS, 0, s>>>(t->view(), t->box, kp, t->qs.q, t->qs.qn, t->qs.perm, m, limit, nIter, r2f);
+    k_ball_wave<FORM, KT><<<grid, NBS, 0, s>>>(t->view(), t->box, kp, t->qs.q, t->qs.qn, t->qs.perm, m, limit, nIter, r2f);
   }
   LAUNCH_CHECK();
   return MLOD_OK;

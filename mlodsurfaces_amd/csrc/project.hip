@@ -139,7 +139,7 @@ template <> struct FormT<1> { using T = float;  using Acc = AccC32; };
 template <> struct FormT<2> { using T = double; using Acc = AccP64; };
 
 struct Counters {
-  uint32_t far_, leafPts, tested, nearN;
+  uint32_t far_, leafPts, tested, nearN, iters;
 };
 
 // Contribution of one point (leaf member) with multiplier f, weight at the point's mean
@@ -356,6 +356,7 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
   T shareScale = SEM == MLOD_SEM_REF_CPU ? tratio<FORM>(T(1), (T)nc) : tratio<FORM>(T(1), (T)Ai);
   bool active = true;
   while (active) {
+    if constexpr (CNT) ct.iters += 1;
     // --- pick X: the next child of the current node, the next near node to expand, or a
     // node with near children still waiting, to restore
     const bool testing = k < nc;
@@ -588,7 +589,7 @@ __global__ __launch_bounds__(PBS) void k_project(TreeView tv, KParams kp, float4
   const float4 q4 = q[i];
   V3<double> qd{q4.x, q4.y, q4.z};
   float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
-  Counters ct{0, 0, 0, 0};
+  Counters ct{0, 0, 0, 0, 0};
   for (uint32_t it = 0; it < nIter; ++it) {
     typename FormT<FORM>::Acc acc;
     acc.init();
@@ -608,6 +609,9 @@ __global__ __launch_bounds__(PBS) void k_project(TreeView tv, KParams kp, float4
     atomicAdd(&counters[1], (unsigned long long)ct.leafPts);
     atomicAdd(&counters[2], (unsigned long long)ct.tested);
     atomicAdd(&counters[3], (unsigned long long)ct.nearN);
+    atomicAdd(&counters[4], (unsigned long long)ct.iters);
+    // per-wavefront maximum of the lanes' loop iterations, one slot per wavefront after the totals
+    atomicMax((unsigned int*)(counters + 5) + (i >> 6), ct.iters);
   }
 }
 
@@ -684,7 +688,7 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
   const float4 q4 = q[i];
   V3<double> qd{q4.x, q4.y, q4.z};
   float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
-  Counters ct{0, 0, 0, 0};
+  Counters ct{0, 0, 0, 0, 0};
   for (uint32_t it = 0; it < nIter; ++it) {
     typename FormT<FORM>::Acc acc;
     acc.init();

@@ -102,3 +102,20 @@ def test_fit_kinds_plane_and_sphere():
         assert np.all(kinds == want), np.bincount(kinds)
         with pytest.raises(A.MlodError):
             ap.fit_kinds(len(q) + 1)
+
+
+def test_count_visits_consistent(setup):
+    """The untimed counter pass: every tested child and every near node made current is one loop
+    iteration, the wavefront slots bound the iterations, and the pass leaves the queries as
+    they were."""
+    p, n, t, K, q = setup
+    ap = A.Fastapss(t, K)
+    ap.set_precision("fp32c")
+    ap.copyPointsToGPU(len(q), q)
+    before = ap.download()
+    c = ap.count_visits()
+    after = ap.download()
+    assert np.array_equal(before[0], after[0])
+    assert c["far"] > 0 and c["tested"] > 0 and c["near"] > 0
+    assert c["iterations"] >= c["tested"] + c["near"]
+    assert c["iterations"] <= c["lane_slots"] <= 64 * c["iterations"]
