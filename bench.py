@@ -292,8 +292,7 @@ def main():
         launch_ms = [ev[s].elapsed_ms(ev[s + 1]) for s in range(args.steps)]
         total_ms = ev[0].elapsed_ms(ev[-1])
         pts, nrm = ap_.download(m, stream=stream.ptr)
-        kinds = ap_.fit_kinds(m, stream=stream.ptr) if mode in ("mlod", "global") and \
-            hasattr(lib_handle(), "mlod_queries_fit_kinds") else None
+        kinds = ap_.fit_kinds(m, stream=stream.ptr) if hasattr(lib_handle(), "mlod_queries_fit_kinds") else None
         results[form] = dict(total_ms=total_ms, wall_s=wall, launch_ms=launch_ms, pts=pts, nrm=nrm, kinds=kinds,
                              pts_per_s=m * args.steps / (total_ms / 1e3))
         log(f"{form}: {args.steps} steps in {total_ms:.1f} ms (wall {wall * 1e3:.1f} ms), "
@@ -305,12 +304,15 @@ def main():
         if ref64 is not None and form != "fp64":
             e = np.linalg.norm(r["pts"].astype(np.float64) - ref64["pts"].astype(np.float64), axis=1) / diag
             ok = np.isfinite(e)
-            errs[form] = dict(max=float(e[ok].max()), rms=float(np.sqrt(np.mean(e[ok] ** 2))), nonfinite=int((~ok).sum()))
+            errs[form] = dict(max=float(e[ok].max()), rms=float(np.sqrt(np.mean(e[ok] ** 2))), nonfinite=int((~ok).sum()),
+                              p9999=float(np.quantile(e[ok], 0.9999)), above_1e6=int(np.sum(e[ok] > 1e-6)))
             if r["kinds"] is not None and ref64["kinds"] is not None:
                 # plane / sphere switch flips of the last step against the FP64 form
                 errs[form]["plane_sphere_flips"] = int(np.sum((r["kinds"] == 1) != (ref64["kinds"] == 1)))
                 errs[form]["plane_fits"] = int(np.sum(r["kinds"] == 1))
-            log(f"{form} vs fp64 after {args.steps} steps: max {errs[form]['max']:.3e} rms {errs[form]['rms']:.3e} diag")
+            log(f"{form} vs fp64 after {args.steps} steps: max {errs[form]['max']:.3e} rms {errs[form]['rms']:.3e} "
+                f"p99.99 {errs[form]['p9999']:.3e} diag, {errs[form]['above_1e6']} queries above 1e-6 diag, "
+                f"{errs[form].get('plane_sphere_flips', '-')} plane/sphere flips")
 
     head = results[args.precision]
     mean_launch_s = float(np.mean(head["launch_ms"])) / 1e3

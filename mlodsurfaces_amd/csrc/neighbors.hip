@@ -178,8 +178,10 @@ __device__ __forceinline__ T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b
 
 // projectOnAPSSStats with q aliased to the output (CPUUtils.cpp:735, 739-782): plane
 // threshold 1e-12, normalize(0) = NaN, NaN positions kept with a zero normal, no step clamp.
+// kind: 0 sphere, 1 plane, 2 invalid (NaN position kept, zero normal)
 template <class T>
-__device__ __forceinline__ void fit_aliased(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T> q, V3<T>& outP, V3<T>& outN) {
+__device__ __forceinline__ void fit_aliased(T sw, V3<T> sp, V3<T> sn, T spn, T spp, V3<T> q, V3<T>& outP, V3<T>& outN,
+                                            float& kind) {
   const T A = spn - dot3(sp, V3<T>{sn.x / sw, sn.y / sw, sn.z / sw});
   const T B = spp - dot3(sp, V3<T>{sp.x / sw, sp.y / sw, sp.z / sw});
   const T u4 = (T)((0.5 * (double)A) / (double)B);
@@ -187,6 +189,7 @@ __device__ __forceinline__ void fit_aliased(T sw, V3<T> sp, V3<T> sn, T spn, T s
   const V3<T> u{(sn.x - t2 * sp.x) / sw, (sn.y - t2 * sp.y) / sw, (sn.z - t2 * sp.z) / sw};
   const T u0 = -(dot3(sp, u) + u4 * spp) / sw;
   V3<T> p = q, n;
+  kind = fabs((double)u4) < 0.000000000001 ? 1.f : 0.f;
   if (fabs((double)u4) < 0.000000000001) {
     const V3<T> nn{-u.x, -u.y, -u.z};
     const T lambda = (u0 - dot3(p, nn)) / dot3(nn, nn);
@@ -205,7 +208,10 @@ __device__ __forceinline__ void fit_aliased(T sw, V3<T> sp, V3<T> sn, T spn, T s
     const T ig = T(1) / sqrt(dot3(g, g));
     n = V3<T>{g.x * ig, g.y * ig, g.z * ig};
   }
-  if (isnan(p.x) || isnan(p.y) || isnan(p.z)) n = V3<T>{0, 0, 0};
+  if (isnan(p.x) || isnan(p.y) || isnan(p.z)) {
+    n = V3<T>{0, 0, 0};
+    kind = 2.f;
+  }
   outP = p;
   outN = n;
 }
@@ -214,20 +220,24 @@ template <int FORM>
 __device__ __forceinline__ void fit_step(const typename Sums<FORM>::S& s, V3<double>& qd, V3<float> qf, float4& nout) {
   if constexpr (FORM == 0) {
     V3<float> p, n;
-    fit_aliased<float>(s.w, V3<float>{s.px, s.py, s.pz}, V3<float>{s.nx, s.ny, s.nz}, s.pn, s.pp, qf, p, n);
+    float kind;
+    fit_aliased<float>(s.w, V3<float>{s.px, s.py, s.pz}, V3<float>{s.nx, s.ny, s.nz}, s.pn, s.pp, qf, p, n, kind);
     qd = V3<double>{p.x, p.y, p.z};
-    nout = make_float4(n.x, n.y, n.z, 0.f);
+    nout = make_float4(n.x, n.y, n.z, kind);
   } else if constexpr (FORM == 1) {
     V3<double> p, n;
+    float kind;
     fit_aliased<double>(s.w.val(), V3<double>{s.px.val(), s.py.val(), s.pz.val()},
-                        V3<double>{s.nx.val(), s.ny.val(), s.nz.val()}, s.pn.val(), s.pp.val(), V3<double>{0, 0, 0}, p, n);
+                        V3<double>{s.nx.val(), s.ny.val(), s.nz.val()}, s.pn.val(), s.pp.val(), V3<double>{0, 0, 0}, p, n,
+                        kind);
     qd = V3<double>{(double)qf.x + p.x, (double)qf.y + p.y, (double)qf.z + p.z};
-    nout = make_float4((float)n.x, (float)n.y, (float)n.z, 0.f);
+    nout = make_float4((float)n.x, (float)n.y, (float)n.z, kind);
   } else {
     V3<double> p, n;
-    fit_aliased<double>(s.w, V3<double>{s.px, s.py, s.pz}, V3<double>{s.nx, s.ny, s.nz}, s.pn, s.pp, qd, p, n);
+    float kind;
+    fit_aliased<double>(s.w, V3<double>{s.px, s.py, s.pz}, V3<double>{s.nx, s.ny, s.nz}, s.pn, s.pp, qd, p, n, kind);
     qd = p;
-    nout = make_float4((float)n.x, (float)n.y, (float)n.z, 0.f);
+    nout = make_float4((float)n.x, (float)n.y, (float)n.z, kind);
   }
 }
 
