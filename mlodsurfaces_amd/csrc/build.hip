@@ -591,7 +591,8 @@ int build_tree(mlod_tree* t, const float* d_xyz, const float* d_nrm, hipStream_t
   }
   lev.v[kMaxDepth + 1] = (uint32_t)tot;
   t->level_off[kMaxDepth + 1] = (uint32_t)tot;
-  if (tot > 0xfffffff0ull) return fail(MLOD_ERR_UNSUPPORTED, "octree has more than 2^32 nodes");
+  // kernels address node records with 32-bit byte offsets (32-byte moment pairs)
+  if (tot >= (1ull << 27)) return fail(MLOD_ERR_UNSUPPORTED, "octree has 2^27 nodes or more");
   t->nn = (uint32_t)tot;
   const uint32_t nn = t->nn;
   uint32_t *nodeLast = nullptr, *pointLeaf = nullptr;

@@ -197,13 +197,21 @@ __device__ __forceinline__ void add_point(typename FormT<FORM>::Acc& acc, const 
 template <int FORM> struct NodeMom { float4 m0, m1; };
 template <> struct NodeMom<2> { double4 m0, m1; double A; };
 
+// Record `idx` of `bytes` bytes from `base`, with the byte offset formed in 32 bits (the tree
+// build guarantees node counts below 2^27), so the address is a scalar base plus one 32-bit
+// vector offset instead of 64-bit vector arithmetic.
+template <class T>
+__device__ __forceinline__ T ld_rec(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx * (uint32_t)sizeof(T)));
+}
+
 template <int FORM>
 __device__ __forceinline__ NodeMom<FORM> load_mom(const TreeView& tv, uint32_t c) {
   NodeMom<FORM> r;
   if constexpr (FORM == 0) {
-    r.m0 = tv.momP[2 * (size_t)c], r.m1 = tv.momP[2 * (size_t)c + 1];
+    r.m0 = ld_rec(tv.momP, 2 * c), r.m1 = ld_rec(tv.momP, 2 * c + 1);
   } else if constexpr (FORM == 1) {
-    r.m0 = tv.momC[2 * (size_t)c], r.m1 = tv.momC[2 * (size_t)c + 1];
+    r.m0 = ld_rec(tv.momC, 2 * c), r.m1 = ld_rec(tv.momC, 2 * c + 1);
   } else {
     r.m0 = tv.momD[2 * (size_t)c], r.m1 = tv.momD[2 * (size_t)c + 1];
     r.A = tv.AD[c];
@@ -406,8 +414,8 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
     float4 gx = make_float4(0.f, 0.f, 0.f, 0.f);
     uint4 hx = make_uint4(0u, 0u, 0u, 0u);
     if (active) {
-      gx = tv.geo[x];
-      hx = tv.hdr[x];
+      gx = ld_rec(tv.geo, x);
+      hx = ld_rec(tv.hdr, x);
     }
     // --- X's blend factor against its parent (CPUUtils.cpp:489-497)
     bool contrib = ownContrib;
