@@ -30,7 +30,6 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
-#include <deque>
 #include <queue>
 #include <mutex>
 #include <thread>
@@ -166,6 +165,7 @@ struct BatchSlot {
   void* sortTmp = nullptr;
   size_t sortBytes = 0;
   std::vector<SReq> reqs;
+  std::vector<uint32_t> chunkOff;  // compact slot offset of each chunk
   uint32_t slots = 0;
   bool busy = false;
   clk::time_point launched;
@@ -253,35 +253,47 @@ void take_requests(mlod_server* S, BatchSlot& b) {
 
 // Stages and launches the requests in b.reqs (worker thread, without the mutex).
 int launch_batch(mlod_server* S, BatchSlot& b) {
-  uint32_t off = 0, blk = 0;
+  // consecutive chunks of one request share a descriptor, so the request's queries are sorted
+  // and walked together
+  uint32_t off = 0, nReq = 0;
+  b.chunkOff.resize(b.reqs.size());
   for (size_t r = 0; r < b.reqs.size(); ++r) {
     const SReq& c = b.reqs[r];
     const Request& q = *c.req;
+    b.chunkOff[r] = off;
+    std::memcpy(b.hIn + 3 * (size_t)off, q.in + 3 * (size_t)c.off, (size_t)c.m * 3 * sizeof(float));
+    const bool extend = r > 0 && b.reqs[r - 1].req == c.req && b.reqs[r - 1].off + b.reqs[r - 1].m == c.off;
+    off += c.m;
+    if (extend) {
+      b.hReq[nReq - 1].count += c.m;
+      continue;
+    }
     const mlod_tree* t = S->trees[q.tree];
-    ReqDesc& d = b.hReq[r];
+    ReqDesc& d = b.hReq[nReq++];
     d.tv = t->view();
     d.w = make_wcoef(q.kernel);
     d.cutoff = q.cutoff;
     d.stepMax = q.stepMax < 0.f ? t->radius : q.stepMax;
     d.minDepth = q.minDepth;
     d.nIter = q.nIter;
-    d.slot0 = off;
+    d.slot0 = b.chunkOff[r];
     d.count = c.m;
-    d.pad0 = blk * kBatchWG;
     for (int a = 0; a < 3; ++a) {
       const float ext = t->BB[a] - t->bb[a];
       d.org[a] = t->bb[a];
       d.scale[a] = ext > 0.f ? 1024.f / ext : 0.f;
     }
-    std::memcpy(b.hIn + 3 * (size_t)off, q.in + 3 * (size_t)c.off, (size_t)c.m * 3 * sizeof(float));
-    const uint32_t nb = (c.m + kBatchWG - 1) / kBatchWG;
-    for (uint32_t i = 0; i < nb; ++i) b.hBlockReq[blk + i] = (uint32_t)r;
-    off += c.m;
+  }
+  uint32_t blk = 0;
+  for (uint32_t r = 0; r < nReq; ++r) {
+    ReqDesc& d = b.hReq[r];
+    d.pad0 = blk * kBatchWG;
+    const uint32_t nb = (d.count + kBatchWG - 1) / kBatchWG;
+    for (uint32_t i = 0; i < nb; ++i) b.hBlockReq[blk + i] = r;
     blk += nb;
   }
   const uint32_t slots = off;
   b.slots = slots;
-  const uint32_t nReq = (uint32_t)b.reqs.size();
   int reqBits = 1;
   while ((1u << reqBits) < nReq) ++reqBits;
   hipError_t e = hipMemcpyAsync(b.dIn, b.hIn, (size_t)slots * 3 * sizeof(float), hipMemcpyHostToDevice, b.s);
@@ -319,7 +331,7 @@ void complete_batch(mlod_server* S, BatchSlot& b, std::vector<mlod_server_done>&
   for (size_t r = 0; r < b.reqs.size(); ++r) {
     const SReq& c = b.reqs[r];
     const Request& q = *c.req;
-    const uint32_t off = b.hReq[r].slot0;
+    const uint32_t off = b.chunkOff[r];
     if (q.outP) std::memcpy(q.outP + 3 * (size_t)c.off, b.hOut + 3 * (size_t)off, (size_t)c.m * 3 * sizeof(float));
     if (q.outN)
       std::memcpy(q.outN + 3 * (size_t)c.off, b.hOut + 3 * ((size_t)S->cap + off), (size_t)c.m * 3 * sizeof(float));
