@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 PEAK_HBM_GBS = 8000.0
-SERVE_BATCH, SERVE_CHUNK, SERVE_INFLIGHT = 262144, 65536, 4
+SERVE_BATCH, SERVE_CHUNK, SERVE_INFLIGHT = 131072, 32768, 3
 
 
 def log(*a):
@@ -179,7 +179,7 @@ def serve_measure(n_clouds=64, seed_rate=2000.0):
     cap_rate = cap["pts_per_s"] / mean_pts
     spec, _, _ = SV.run_load(srv, trees, clouds, rate=seed_rate, seconds=0.5, seed=12, log=log)
     sus_rate = 0.7 * cap_rate
-    sus, _, _ = SV.run_load(srv, trees, clouds, rate=sus_rate, seconds=3.0, seed=13, log=log)
+    sus, _, _ = SV.run_load(srv, trees, clouds, rate=sus_rate, seconds=6.0, seed=13, log=log)
     srv.close()
     keys = ("requests", "offered_rate", "pts_per_s", "p50_ms", "p99_ms", "p50_100k_ms", "p99_100k_ms", "n_100k", "wall_s")
     return {"clouds": len(clouds), "resident_points": npts, "max_batch": SERVE_BATCH, "chunk": SERVE_CHUNK,

@@ -157,8 +157,8 @@ int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out6, void* stream);
  * blocking calls, scene.cpp:916-966).  A server keeps registered octrees resident, takes
  * projection requests from any thread, cuts them into chunks of at most chunk_queries queries,
  * packs pending chunks into batches of at most max_batch_queries queries, earliest deadline first
- * (arrival time plus the request's own projected-point count at a nominal rate: equal requests in
- * arrival order, small ones not stuck behind large ones), and keeps up to `inflight` batches on
+ * (arrival time plus 1 ms per 20k projected points of the request: equal requests in arrival
+ * order, small ones ahead of large ones), and keeps up to `inflight` batches on
  * the device at once (continuous batching).  Each query gives the same result alone or in any
  * batch.  Registered trees must outlive the server; request buffers must stay valid until the
  * request's completion is polled. */

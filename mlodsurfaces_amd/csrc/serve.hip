@@ -7,10 +7,10 @@
 // (device memory per batch is fixed at creation) and keeps up to `inflight` batches on separate
 // streams, so requests that arrive while others are on the GPU join the next batch as soon as a
 // batch slot frees, and the batches' copies and kernels overlap on the device.  Chunks are taken
-// earliest deadline first, the deadline being the arrival time plus the request's own service
-// time at the nominal rate (kNominalPtsPerMs projected points per ms): requests of equal size
-// are served in arrival order, and a small request that arrives behind a large one does not
-// wait for all of it.
+// earliest deadline first, the deadline being the arrival time plus 1 ms per kNominalPtsPerMs
+// projected points of the request (about ten times its service time on an idle GPU): requests
+// of equal size are served in arrival order, a small request that arrives behind large ones
+// overtakes them, and a large one is not starved (its deadline still comes).
 //
 // One batch on its stream: pinned host staging -> device copy; per-slot Morton key in the
 // request's tree box with the request index above it; radix sort (so each wavefront walks one
@@ -43,7 +43,7 @@ namespace {
 using clk = std::chrono::steady_clock;
 constexpr int BS = 256;
 constexpr uint32_t kMaxReqPerBatch = 1024;  // chunk index fits 10 bits above the 30-bit Morton code
-constexpr double kNominalPtsPerMs = 2.0e5;  // deadline slack per projected point (200 M pts/s)
+constexpr double kNominalPtsPerMs = 2.0e4;  // deadline slack: 1 ms per 20k projected points
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   v &= 0x3ffu;

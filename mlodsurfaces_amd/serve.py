@@ -33,7 +33,7 @@ class Server:
     most ``max_batch`` queries on the GPU at once; requests are cut into chunks of ``chunk``
     queries (0: max_batch / 4)."""
 
-    def __init__(self, max_batch=262144, inflight=4, semantics="ref_gpu", precision="fp32c", chunk=0):
+    def __init__(self, max_batch=131072, inflight=3, semantics="ref_gpu", precision="fp32c", chunk=32768):
         sem = {"ref_gpu": 0, "ref_cpu": 1}[semantics]
         prec = {"fp32": 0, "fp32c": 1, "fp64": 2}[precision]
         h = C.c_void_p()
@@ -204,9 +204,9 @@ def main():
     ap.add_argument("--rate", type=float, nargs="+", default=[2000.0])
     ap.add_argument("--seconds", type=float, default=2.0)
     ap.add_argument("--clouds", type=int, default=64)
-    ap.add_argument("--max-batch", type=int, default=262144)
-    ap.add_argument("--chunk", type=int, default=65536)
-    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--max-batch", type=int, default=131072)
+    ap.add_argument("--chunk", type=int, default=32768)
+    ap.add_argument("--inflight", type=int, default=3)
     ap.add_argument("--max-pts", type=int, default=4_000_000)
     args = ap.parse_args()
     log = lambda *a: print("[serve]", *a, file=sys.stderr, flush=True)  # noqa: E731
