@@ -33,7 +33,7 @@ class Server:
     most ``max_batch`` queries on the GPU at once; requests are cut into chunks of ``chunk``
     queries (0: max_batch / 4)."""
 
-    def __init__(self, max_batch=131072, inflight=3, semantics="ref_gpu", precision="fp32c", chunk=32768):
+    def __init__(self, max_batch=131072, inflight=3, semantics="ref_gpu", precision="fp32c", chunk=0):
         sem = {"ref_gpu": 0, "ref_cpu": 1}[semantics]
         prec = {"fp32": 0, "fp32c": 1, "fp64": 2}[precision]
         h = C.c_void_p()
