@@ -155,18 +155,22 @@ int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out6, void* stream);
 /* ---------------------------------------------------------------------------------------
  * Serving.  No counterpart in the reference (its harness projects one batch at a time with
  * blocking calls, scene.cpp:916-966).  A server keeps registered octrees resident, takes
- * projection requests from any thread, cuts them into chunks of at most chunk_queries queries,
- * packs pending chunks into batches of at most max_batch_queries queries, earliest deadline first
- * (arrival time plus 1 ms per 20k projected points of the request: equal requests in arrival
- * order, small ones ahead of large ones), and keeps up to `inflight` batches on
- * the device at once (continuous batching).  Each query gives the same result alone or in any
- * batch.  Registered trees must outlive the server; request buffers must stay valid until the
- * request's completion is polled. */
+ * projection requests from any thread and schedules them a projection step at a time
+ * (continuous batching): requests are cut into chunks of at most chunk_queries queries and
+ * queued by class (interactive: at most 2M projected points, ahead of bulk, which still gets a
+ * launch at least every 20 ms) and deadline (arrival time plus 1 ms per 20k projected points);
+ * each of `inflight` streams keeps one step launch of at most max_batch_queries queries on the
+ * device and, when it completes, takes the most urgent resident or queued work for the next one.
+ * Queries stay in a device pool between steps; bulk launches take two steps, interactive ones
+ * up to 16 at raised wavefront priority; a request's results reach its buffers as soon as its
+ * own workgroups finish.  Each query gives the same result alone, in any mix and through
+ * mlod_project.  Registered trees must outlive the server;
+ * request buffers must stay valid until the request's completion is polled. */
 typedef struct mlod_server mlod_server;
 typedef struct {
   uint64_t ticket;
   double latency_ms;   /* submit -> all results copied to the caller's buffers */
-  double queue_ms;     /* submit -> first chunk's batch launched */
+  double queue_ms;     /* submit -> first step launch of the request */
   double completed_s;  /* completion time since server creation */
 } mlod_server_done;
 
@@ -177,13 +181,14 @@ int mlod_server_destroy(mlod_server* s);
 int mlod_server_add_tree(mlod_server* s, mlod_tree* t, uint32_t* tree_id);
 /* n_iter projection steps of m queries (host, m x 3, any m >= 1) on tree tree_id with kernel k;
  * results go to out_pts / out_nmls (host, m x 3, either may be NULL).  step_max < 0 => the tree's
- * radius.  Multi-Gaussian kernels take at most 8 sigmas here. */
+ * radius.  Multi-Gaussian kernels take at most 8 sigmas here.  n_iter = 0 completes at once with
+ * the queries unchanged and zero normals. */
 int mlod_server_submit(mlod_server* s, uint32_t tree_id, const mlod_kernel* k, float cutoff, uint32_t min_depth,
                        uint32_t n_iter, float step_max, const float* queries, uint32_t m, float* out_pts,
                        float* out_nmls, uint64_t* ticket);
 /* Completed requests since the last poll (waits up to timeout_s seconds for one). */
 int mlod_server_poll(mlod_server* s, mlod_server_done* out, uint32_t cap, uint32_t* n_out, double timeout_s);
-/* batches launched, requests completed, queries launched, requests pending */
+/* step launches, requests completed, queries completed, chunks queued */
 int mlod_server_stats(mlod_server* s, uint64_t* out4);
 
 /* ---------------------------------------------------------------------------------------

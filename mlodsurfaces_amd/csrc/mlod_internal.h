@@ -78,19 +78,34 @@ struct WCoef {
   float b[kServeMaxSigmas];
 };
 
-// One request of a serving batch: its tree, kernel, parameters and slot range.
+// One run of a serving step launch (a resident range of one request's queries): its tree,
+// kernel, parameters and place in the server's query pool.
 struct ReqDesc {
   TreeView tv;
   WCoef w;
   float cutoff;
   float stepMax;
   uint32_t minDepth;
-  uint32_t nIter;
-  uint32_t slot0;   // first slot of the request in the batch (requests packed back to back)
+  uint32_t nIter;   // steps in this launch
+  uint32_t slot0;   // first slot of the run in the admission staging (runs packed back to back)
   uint32_t count;   // queries
-  uint32_t pad0;    // first slot of the request in the workgroup-aligned layout
+  uint32_t pad0;    // first pool slot of the run (a multiple of kBatchWG)
+  uint32_t outOff;  // last steps: first result row of the run in the launch's host staging; else kNoOut
   float org[3];     // tree box origin and Morton scale
   float scale[3];
+};
+constexpr uint32_t kNoOut = 0xffffffffu;
+
+// Where the runs taking their last step put their results: rows outOff + (index within the run,
+// pidx of the pool slot) of `out` (positions, then normals from row cap on), in pinned,
+// host-coherent memory; each workgroup of such a run then stores `seq` to flag[its launch block],
+// so the host sees a run finish while the launch still runs others.
+struct BatchOut {
+  const uint32_t* pidx;
+  float* out;
+  uint32_t cap;
+  uint32_t* flag;
+  uint32_t seq;
 };
 
 struct QueryState {
@@ -179,11 +194,13 @@ KParams make_kparams(const mlod_kernel& k);
 int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s);
 int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream_t s);
 
-// serving batch (project.hip): the workgroup-aligned slots of workgroups [blk0, blk1) (each
-// workgroup of kBatchWG slots belongs to request blockReq[block]), all kernel types in one launch
+// serving step launches (project.hip): the server's query pool is cut into blocks of kBatchWG
+// slots, one workgroup each; runs of every kernel type share one launch
 constexpr uint32_t kBatchWG = 128;
-int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_blockReq, float4* d_q, float4* d_qn, uint32_t blk0,
-                         uint32_t blk1, int semantics, int precision, hipStream_t s);
+// One launch over nblk pool blocks: block b holds queries [d_blocks[b].y * kBatchWG, +kBatchWG) of run
+// d_reqs[d_blocks[b].x] (those below its pad0 + count); prio != 0 raises the wavefronts' priority.
+int launch_project_batch(const ReqDesc* d_reqs, const uint2* d_blocks, uint32_t nblk, float4* d_q, float4* d_qn,
+                         const BatchOut& out, uint32_t prio, int semantics, int precision, hipStream_t s);
 WCoef make_wcoef(const mlod_kernel& k);
 
 // neighbors.hip

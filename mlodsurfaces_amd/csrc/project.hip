@@ -670,48 +670,71 @@ __global__ __launch_bounds__(PBS) void k_global(TreeView tv, KParams kp, float4*
   }
 }
 
-// Serving batch: workgroup b of the aligned layout projects kBatchWG slots of request
-// blockReq[b] with that request's tree, kernel and parameters, read once per workgroup into
-// scalar registers; per-lane work and results are those of k_project, so a query gives the same
-// bits alone or in any batch.
+// Serving step launch: workgroup b projects the kBatchWG pool slots of pool block blocks[b].y
+// with the tree, kernel and parameters of run descriptor blocks[b].x, read once per workgroup
+// into scalar registers; per-lane work and results are those of k_project, so a query gives the
+// same bits alone, in any mix, and whether its steps run in one launch or several.  A launch for
+// interactive requests raises its wavefronts' issue priority (s_setprio), so they keep close to
+// their unloaded speed beside bulk launches.  A run taking its last steps writes its results to
+// the host staging and flags each of its workgroups done (system-scope release).
 template <int FORM, int SEM, int KT>
-__global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict__ reqs, const uint32_t* __restrict__ blockReq,
-                                                       float4* __restrict__ q, float4* __restrict__ qn, uint32_t blk0) {
-  static_assert(PBS == kBatchWG, "one workgroup per aligned request block");
+__global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict__ reqs, const uint2* __restrict__ blocks,
+                                                       float4* __restrict__ q, float4* __restrict__ qn, BatchOut bo,
+                                                       uint32_t prio) {
+  static_assert(PBS == kBatchWG, "one workgroup per pool block");
   using T = typename FormT<FORM>::T;
   __shared__ uint32_t sNode[kLevels][PBS];
   __shared__ uint8_t sMask[kLevels][PBS];
   __shared__ T sMul[kLevels][PBS];
+  if (prio) __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x;
-  const uint32_t blk = blk0 + xcd_block();
-  const uint32_t r = __builtin_amdgcn_readfirstlane(blockReq[blk]);
+  const uint32_t lb = xcd_block();
+  const uint2 b = blocks[lb];
+  const uint32_t r = __builtin_amdgcn_readfirstlane(b.x), pb = __builtin_amdgcn_readfirstlane(b.y);
   const ReqDesc& R = reqs[r];
-  const uint32_t i = blk * PBS + tid;
-  if (i >= R.pad0 + R.count) return;
-  const TreeView tv = R.tv;
-  const WCoef w = R.w;
-  const float cutoff = R.cutoff, stepMax = R.stepMax;
-  const uint32_t minDepth = R.minDepth, nIter = R.nIter;
-  const LevelState<T> st{sNode, sMask, sMul};
-  const float4 q4 = q[i];
-  V3<double> qd{q4.x, q4.y, q4.z};
-  float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
-  Counters ct{0, 0, 0, 0, 0};
-  for (uint32_t it = 0; it < nIter; ++it) {
-    typename FormT<FORM>::Acc acc;
-    acc.init();
-    const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
-    const V3<T> qt{qf.x, qf.y, qf.z};
-    traverse<FORM, SEM, KT, false>(acc, w, tv, qt, (T)cutoff, minDepth, st, tid, ct);
-    fit_step<FORM, SEM>(acc, qd, qf, stepMax, nout);
+  const uint32_t i = pb * PBS + tid, outOff = R.outOff;
+  if (i < R.pad0 + R.count) {
+    const TreeView tv = R.tv;
+    const WCoef w = R.w;
+    const float cutoff = R.cutoff, stepMax = R.stepMax;
+    const uint32_t minDepth = R.minDepth, nIter = R.nIter;
+    const LevelState<T> st{sNode, sMask, sMul};
+    const float4 q4 = q[i];
+    V3<double> qd{q4.x, q4.y, q4.z};
+    float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
+    Counters ct{0, 0, 0, 0, 0};
+    for (uint32_t it = 0; it < nIter; ++it) {
+      typename FormT<FORM>::Acc acc;
+      acc.init();
+      const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
+      const V3<T> qt{qf.x, qf.y, qf.z};
+      traverse<FORM, SEM, KT, false>(acc, w, tv, qt, (T)cutoff, minDepth, st, tid, ct);
+      fit_step<FORM, SEM>(acc, qd, qf, stepMax, nout);
+    }
+    const float4 p4 = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
+    q[i] = p4;
+    qn[i] = nout;
+    if (outOff != kNoOut) {
+      const uint64_t d = outOff + bo.pidx[i], e = (uint64_t)bo.cap + d;
+      bo.out[3 * d] = p4.x;
+      bo.out[3 * d + 1] = p4.y;
+      bo.out[3 * d + 2] = p4.z;
+      bo.out[3 * e] = nout.x;
+      bo.out[3 * e + 1] = nout.y;
+      bo.out[3 * e + 2] = nout.z;
+    }
   }
-  q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
-  qn[i] = nout;
+  if (outOff != kNoOut) {  // uniform over the workgroup
+    __threadfence_system();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(bo.flag + lb, bo.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 template <int FORM, int SEM>
-void launch_batch_all(const ReqDesc* r, const uint32_t* br, float4* q, float4* qn, uint32_t b0, uint32_t b1, hipStream_t s) {
-  k_project_batch<FORM, SEM, -1><<<b1 - b0, PBS, 0, s>>>(r, br, q, qn, b0);
+void launch_batch_all(const ReqDesc* r, const uint2* blocks, uint32_t nblk, float4* q, float4* qn, const BatchOut& bo,
+                      uint32_t prio, hipStream_t s) {
+  k_project_batch<FORM, SEM, -1><<<nblk, PBS, 0, s>>>(r, blocks, q, qn, bo, prio);
 }
 
 template <int FORM, int SEM, int KT, bool CNT>
@@ -791,16 +814,16 @@ WCoef make_wcoef(const mlod_kernel& k) {
   return w;
 }
 
-int launch_project_batch(const ReqDesc* d_reqs, const uint32_t* d_blockReq, float4* d_q, float4* d_qn, uint32_t blk0,
-                         uint32_t blk1, int semantics, int precision, hipStream_t s) {
-  if (blk1 <= blk0) return MLOD_OK;
+int launch_project_batch(const ReqDesc* d_reqs, const uint2* d_blocks, uint32_t nblk, float4* d_q, float4* d_qn,
+                         const BatchOut& bo, uint32_t prio, int semantics, int precision, hipStream_t s) {
+  if (nblk == 0) return MLOD_OK;
   const bool cpu = semantics == MLOD_SEM_REF_CPU;
   if (precision == MLOD_PREC_FP32C) {
-    if (cpu) launch_batch_all<1, MLOD_SEM_REF_CPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, s);
-    else launch_batch_all<1, MLOD_SEM_REF_GPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, s);
+    if (cpu) launch_batch_all<1, MLOD_SEM_REF_CPU>(d_reqs, d_blocks, nblk, d_q, d_qn, bo, prio, s);
+    else launch_batch_all<1, MLOD_SEM_REF_GPU>(d_reqs, d_blocks, nblk, d_q, d_qn, bo, prio, s);
   } else if (precision == MLOD_PREC_FP32) {
-    if (cpu) launch_batch_all<0, MLOD_SEM_REF_CPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, s);
-    else launch_batch_all<0, MLOD_SEM_REF_GPU>(d_reqs, d_blockReq, d_q, d_qn, blk0, blk1, s);
+    if (cpu) launch_batch_all<0, MLOD_SEM_REF_CPU>(d_reqs, d_blocks, nblk, d_q, d_qn, bo, prio, s);
+    else launch_batch_all<0, MLOD_SEM_REF_GPU>(d_reqs, d_blocks, nblk, d_q, d_qn, bo, prio, s);
   } else {
     return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
   }

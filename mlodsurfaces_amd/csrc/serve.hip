@@ -2,26 +2,37 @@
 //
 // The reference has no server: its harness projects one query batch at a time with blocking
 // calls (scene.cpp:916-966).  Here clients submit projection requests (tree, kernel,
-// parameters, queries, n_iter) from any thread.  A request is cut into chunks of at most
-// `chunk` queries; a worker thread packs pending chunks into batches bounded by a slot budget
-// (device memory per batch is fixed at creation) and keeps up to `inflight` batches on separate
-// streams, so requests that arrive while others are on the GPU join the next batch as soon as a
-// batch slot frees, and the batches' copies and kernels overlap on the device.  Chunks are taken
-// earliest deadline first, the deadline being the arrival time plus 1 ms per kNominalPtsPerMs
-// projected points of the request (about ten times its service time on an idle GPU): requests
-// of equal size are served in arrival order, a small request that arrives behind large ones
-// overtakes them, and a large one is not starved (its deadline still comes).
+// parameters, queries, n_iter) from any thread, and the server schedules them a projection step
+// at a time:
 //
-// One batch on its stream: pinned host staging -> device copy; per-slot Morton key in the
-// request's tree box with the request index above it; radix sort (so each wavefront walks one
-// request's neighbourhood); gather into a layout where every request starts on a workgroup
-// boundary; one projection launch over all workgroups (k_project_batch: the workgroup's request
-// descriptor, kernel type included, is uniform and read into scalar registers; per-query code
-// and bits are those of a standalone projection); scatter back to
-// submission order; device -> pinned copy; completion event.  The worker copies each request's
-// results to the caller's buffers when the event has fired and stamps the completion time.
-// Only the request queue and the completion list are shared with callers (under the mutex);
-// staging copies and launches run outside it.
+// * Queueing.  A request is cut into chunks of at most `chunk` queries.  Interactive requests
+//   (at most kInteractivePts projected points) go ahead of bulk ones; within a class the
+//   earliest deadline goes first (arrival + 1 ms per kNominalPtsPerMs projected points), then
+//   arrival order.  Bulk work gets a launch at least every kBulkMaxWaitMs, so interactive
+//   requests cannot starve it.
+// * Residency.  Chunks taken by a step launch are staged into a device query pool (pinned host
+//   staging -> device; Morton key in the tree's box; radix sort; gather into pool blocks of
+//   kBatchWG slots) and stay there as a run until their last step; consecutive chunks of one
+//   request taken together form one run, sorted and walked together.
+// * Steps.  Each of `inflight` streams keeps one step launch on the device; when it completes,
+//   the worker fills the stream's next launch with up to `max_batch_queries` queries of the most
+//   urgent runs, resident or queued, of one class.  Bulk launches take kBulkSteps steps per run,
+//   so bulk work yields to interactive work every few milliseconds; interactive launches take up
+//   to kInteractiveSteps (a whole request, at the usual step counts) and raise their
+//   wavefronts' priority.  A request that arrives while others are on the GPU therefore starts
+//   at the next launch boundary of any stream; a run not picked for a launch waits in the pool
+//   at no cost.
+// * Hand-back.  A run taking its last steps writes its results straight to the stream's pinned,
+//   host-coherent staging (rows in submission order) and flags each of its workgroups done; the
+//   worker polls the flags and copies a run to its caller as soon as the run is done, without
+//   waiting for the rest of its launch.
+//
+// A query gives the bits of a standalone projection whatever the mix and the split of its steps
+// over launches: a lane's traversal and fit depend only on its query and tree, and the position
+// carried from one step to the next is the float the next step starts from.  Only the queues,
+// the tree list and the completion list are shared with callers (under the mutex); the pool,
+// runs and streams belong to the worker thread, and staging copies and launches run outside the
+// mutex.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -30,8 +41,9 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
-#include <queue>
+#include <map>
 #include <mutex>
+#include <queue>
 #include <thread>
 #include <vector>
 
@@ -42,8 +54,13 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 constexpr int BS = 256;
-constexpr uint32_t kMaxReqPerBatch = 1024;  // chunk index fits 10 bits above the 30-bit Morton code
+constexpr uint32_t kMaxRuns = 1024;         // runs per launch: the run index fits 10 bits above the Morton code
 constexpr double kNominalPtsPerMs = 2.0e4;  // deadline slack: 1 ms per 20k projected points
+constexpr double kInteractivePts = 2.0e6;   // interactive class: at most 2M projected points (200k queries x 10 steps)
+constexpr uint32_t kInteractiveSteps = 16;  // steps per launch for interactive runs (all, for n_iter <= 16)
+constexpr uint32_t kBulkSteps = 2;          // steps per launch for bulk runs
+constexpr uint32_t kPoolLaunches = 4;       // pool capacity, in launches' worth of blocks per stream
+constexpr double kBulkMaxWaitMs = 20.0;     // bulk work gets a launch at least this often
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   v &= 0x3ffu;
@@ -54,13 +71,14 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   return v;
 }
 
-// key = request index << 30 | Morton code of the query in its tree's box
+// key = run index << 30 | Morton code of the query in its tree's box, over the staged queries of
+// the launch's new runs (reqs[0..nReq), staged back to back from their slot0)
 __global__ __launch_bounds__(BS) void k_batch_keys(const float* __restrict__ in, uint32_t slots,
                                                    const ReqDesc* __restrict__ reqs, uint32_t nReq,
                                                    uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
   const uint32_t i = blockIdx.x * BS + threadIdx.x;
   if (i >= slots) return;
-  uint32_t lo = 0, hi = nReq;  // last request with slot0 <= i
+  uint32_t lo = 0, hi = nReq;  // last run with slot0 <= i
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
     if (reqs[mid].slot0 <= i)
@@ -80,38 +98,19 @@ __global__ __launch_bounds__(BS) void k_batch_keys(const float* __restrict__ in,
   idx[i] = i;
 }
 
-// sorted slot j (request key[j] >> 30) -> its place in the workgroup-aligned layout
-__device__ __forceinline__ uint32_t aligned_slot(const ReqDesc* __restrict__ reqs, uint64_t key, uint32_t j) {
-  const ReqDesc& R = reqs[(uint32_t)(key >> 30)];
-  return R.pad0 + (j - R.slot0);
-}
-
-__global__ __launch_bounds__(BS) void k_batch_gather(const float* __restrict__ in, const uint64_t* __restrict__ key,
-                                                     const uint32_t* __restrict__ perm, uint32_t slots,
-                                                     const ReqDesc* __restrict__ reqs, float4* __restrict__ q,
-                                                     float4* __restrict__ qn) {
+// sorted staged query j -> its pool slot, recording its index within the run
+__global__ __launch_bounds__(BS) void k_pool_gather(const float* __restrict__ in, const uint64_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ perm, uint32_t slots,
+                                                    const ReqDesc* __restrict__ reqs, float4* __restrict__ q,
+                                                    float4* __restrict__ qn, uint32_t* __restrict__ pidx) {
   const uint32_t j = blockIdx.x * BS + threadIdx.x;
   if (j >= slots) return;
-  const uint32_t s = perm[j], d = aligned_slot(reqs, key[j], j);
+  const uint32_t s = perm[j];
+  const ReqDesc& R = reqs[(uint32_t)(key[j] >> 30)];
+  const uint32_t d = R.pad0 + (j - R.slot0);
   q[d] = make_float4(in[3 * (uint64_t)s], in[3 * (uint64_t)s + 1], in[3 * (uint64_t)s + 2], 0.f);
   qn[d] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-// out = [cap x 3 positions | cap x 3 normals] in submission order
-__global__ __launch_bounds__(BS) void k_batch_scatter(const float4* __restrict__ q, const float4* __restrict__ qn,
-                                                      const uint64_t* __restrict__ key, const uint32_t* __restrict__ perm,
-                                                      const ReqDesc* __restrict__ reqs, uint32_t slots, uint32_t cap,
-                                                      float* __restrict__ out) {
-  const uint32_t j = blockIdx.x * BS + threadIdx.x;
-  if (j >= slots) return;
-  const uint32_t d = perm[j], a = aligned_slot(reqs, key[j], j);
-  const float4 p = q[a], n = qn[a];
-  out[3 * (uint64_t)d] = p.x;
-  out[3 * (uint64_t)d + 1] = p.y;
-  out[3 * (uint64_t)d + 2] = p.z;
-  out[3 * ((uint64_t)cap + d)] = n.x;
-  out[3 * ((uint64_t)cap + d) + 1] = n.y;
-  out[3 * ((uint64_t)cap + d) + 2] = n.z;
+  pidx[d] = s - R.slot0;
 }
 
 // One request as submitted.
@@ -127,62 +126,82 @@ struct Request {
   float* outP;
   float* outN;
   uint32_t m;
-  uint32_t chunksLeft;  // chunks not yet completed (worker thread only)
+  uint32_t left;  // queries not yet handed back (worker thread only)
+  bool interactive;
+  bool started;
+  double deadline;  // ms since server creation
   clk::time_point submitted, launched;
 };
 
-// A chunk: queries [off, off + m) of a request.
-struct SReq {
+// Queries [off, off + m) of a request, queued.
+struct Chunk {
   Request* req;
   uint32_t off, m;
-  double deadline;  // ms since server creation
-  uint64_t seq;     // submission order, ties
+  uint64_t seq;  // submission order
 };
 
 struct ChunkLater {
-  bool operator()(const SReq& a, const SReq& b) const {
-    return a.deadline != b.deadline ? a.deadline > b.deadline : a.seq > b.seq;
+  bool operator()(const Chunk& a, const Chunk& b) const {
+    return a.req->deadline != b.req->deadline ? a.req->deadline > b.req->deadline : a.seq > b.seq;
   }
 };
+using ChunkQueue = std::priority_queue<Chunk, std::vector<Chunk>, ChunkLater>;
 
-struct BatchSlot {
+// Queries [off, off + m) of a request, resident in pool blocks [blk0, blk0 + nblk).
+struct Run {
+  Request* req;
+  uint32_t off, m;
+  uint64_t seq;
+  uint32_t blk0, nblk;
+  uint32_t stepsLeft;
+  uint32_t steps;  // steps in the launch that holds it
+  int stream;      // stream whose launch holds it, -1 while it waits in the pool
+  bool fresh;      // not staged yet: its launch stages it first
+  bool handed;     // results handed back (its launch may still run others)
+};
+
+struct Stream {
   hipStream_t s = nullptr;
   hipEvent_t ev = nullptr;
+  // staging of new runs
   float* hIn = nullptr;
-  float* hOut = nullptr;
-  ReqDesc* hReq = nullptr;
   float* dIn = nullptr;
-  float* dOut = nullptr;
-  float4* dQ = nullptr;
-  float4* dQn = nullptr;
   uint64_t* dKey = nullptr;
   uint64_t* dKey2 = nullptr;
   uint32_t* dIdx = nullptr;
   uint32_t* dPerm = nullptr;
-  uint32_t* dBlockReq = nullptr;   // request of each workgroup of the aligned layout
-  uint32_t* hBlockReq = nullptr;
-  ReqDesc* dReq = nullptr;
   void* sortTmp = nullptr;
   size_t sortBytes = 0;
-  std::vector<SReq> reqs;
-  std::vector<uint32_t> chunkOff;  // compact slot offset of each chunk
-  uint32_t slots = 0;
+  // launch tables
+  ReqDesc* hReq = nullptr;
+  ReqDesc* dReq = nullptr;
+  uint2* hBlk = nullptr;
+  uint2* dBlk = nullptr;
+  // results of the runs taking their last steps, written by the launch (pinned, host-coherent):
+  // rows [0, cap) positions, [cap, 2 cap) normals; flag[launch block] = seq when the block is done
+  float* hOut = nullptr;
+  float* dOut = nullptr;  // device address of hOut
+  uint32_t* hFlag = nullptr;
+  uint32_t* dFlag = nullptr;
+  uint32_t seq = 0;
+  std::vector<Run*> runs;         // runs of the launch
+  std::vector<uint32_t> outOff;   // per run: its rows in hOut, kNoOut if it is not handed back
+  std::vector<uint32_t> blk0;     // per run: its first launch block
+  std::vector<uint32_t> seen;     // per run: launch blocks seen done so far
   bool busy = false;
-  clk::time_point launched;
 };
 
 }  // namespace
 }  // namespace mlod
 
 struct mlod_server {
-  uint32_t cap = 0;
+  uint32_t cap = 0;  // queries per launch
   int semantics = MLOD_SEM_REF_GPU;
   int precision = MLOD_PREC_FP32C;
   int device = 0;
   std::vector<mlod_tree*> trees;
-  std::vector<mlod::BatchSlot> slots;
   uint32_t chunk = 0;
-  std::priority_queue<mlod::SReq, std::vector<mlod::SReq>, mlod::ChunkLater> pending;
+  mlod::ChunkQueue qInt, qBulk;  // queued chunks per class
   uint64_t nextSeq = 0;
   std::vector<mlod_server_done> done;
   std::mutex mu;
@@ -192,8 +211,17 @@ struct mlod_server {
   std::atomic<int> error{0};
   std::string errmsg;
   uint64_t nextTicket = 1;
-  uint64_t batches = 0, queries = 0, requests = 0;
+  uint64_t launches = 0, queries = 0, requests = 0;
   mlod::clk::time_point t0 = mlod::clk::now();
+  // worker thread only
+  double lastBulkMs = 0.0;
+  std::vector<mlod::Stream> streams;
+  std::vector<mlod::Run*> runs;  // resident runs
+  float4* dQ = nullptr;          // query pool
+  float4* dQn = nullptr;
+  uint32_t* dPidx = nullptr;
+  uint32_t poolBlocks = 0, freeBlocks = 0;
+  std::map<uint32_t, uint32_t> freeRanges;  // first block -> length
 };
 
 namespace mlod {
@@ -208,138 +236,307 @@ hipError_t hmalloc(T** p, size_t n) {
   return hipHostMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault);
 }
 
-void free_slot(BatchSlot& b) {
+inline uint32_t blocks_of(uint32_t m) { return (m + kBatchWG - 1) / kBatchWG; }
+// blocks of one launch: every run wastes less than one block
+inline uint32_t max_launch_blocks(uint32_t cap) { return blocks_of(cap) + kMaxRuns; }
+
+void free_stream(Stream& b) {
   if (b.s) (void)hipStreamSynchronize(b.s);
-  void* d[] = {b.dIn, b.dOut, b.dQ, b.dQn, b.dKey, b.dKey2, b.dIdx, b.dPerm, b.dBlockReq, b.dReq, b.sortTmp};
+  void* d[] = {b.dIn, b.dKey, b.dKey2, b.dIdx, b.dPerm, b.sortTmp, b.dReq, b.dBlk};
   for (void* p : d)
     if (p) (void)hipFree(p);
-  void* h[] = {b.hIn, b.hOut, b.hReq, b.hBlockReq};
+  void* h[] = {b.hIn, b.hReq, b.hBlk, b.hOut, b.hFlag};
   for (void* p : h)
     if (p) (void)hipHostFree(p);
   if (b.ev) (void)hipEventDestroy(b.ev);
   if (b.s) (void)hipStreamDestroy(b.s);
-  b = BatchSlot{};
+  b = Stream{};
 }
 
-// slots of the workgroup-aligned layout: every request may waste up to kBatchWG - 1
-inline size_t aligned_cap(uint32_t cap) { return (size_t)cap + (size_t)kMaxReqPerBatch * kBatchWG; }
-
-hipError_t alloc_slot(BatchSlot& b, uint32_t cap) {
+hipError_t alloc_stream(Stream& b, uint32_t cap) {
   hipError_t e;
-  const size_t acap = aligned_cap(cap), nblk = acap / kBatchWG;
+  const size_t nblk = max_launch_blocks(cap);
   if ((e = hipStreamCreate(&b.s)) || (e = hipEventCreateWithFlags(&b.ev, hipEventDisableTiming))) return e;
-  if ((e = hmalloc(&b.hIn, 3 * (size_t)cap)) || (e = hmalloc(&b.hOut, 6 * (size_t)cap)) ||
-      (e = hmalloc(&b.hReq, kMaxReqPerBatch)) || (e = hmalloc(&b.hBlockReq, nblk)))
+  if ((e = hmalloc(&b.hIn, 3 * (size_t)cap)) || (e = hmalloc(&b.hReq, kMaxRuns)) || (e = hmalloc(&b.hBlk, nblk)))
     return e;
-  if ((e = dmalloc(&b.dIn, 3 * (size_t)cap)) || (e = dmalloc(&b.dOut, 6 * (size_t)cap)) || (e = dmalloc(&b.dQ, acap)) ||
-      (e = dmalloc(&b.dQn, acap)) || (e = dmalloc(&b.dKey, cap)) || (e = dmalloc(&b.dKey2, cap)) ||
-      (e = dmalloc(&b.dIdx, cap)) || (e = dmalloc(&b.dPerm, cap)) || (e = dmalloc(&b.dBlockReq, nblk)) ||
-      (e = dmalloc(&b.dReq, kMaxReqPerBatch)))
+  if ((e = hipHostMalloc((void**)&b.hOut, 6 * (size_t)cap * sizeof(float), hipHostMallocCoherent)) ||
+      (e = hipHostMalloc((void**)&b.hFlag, nblk * sizeof(uint32_t), hipHostMallocCoherent)))
+    return e;
+  std::memset(b.hFlag, 0, nblk * sizeof(uint32_t));
+  if ((e = hipHostGetDevicePointer((void**)&b.dOut, b.hOut, 0)) ||
+      (e = hipHostGetDevicePointer((void**)&b.dFlag, b.hFlag, 0)))
+    return e;
+  if ((e = dmalloc(&b.dIn, 3 * (size_t)cap)) || (e = dmalloc(&b.dKey, cap)) || (e = dmalloc(&b.dKey2, cap)) ||
+      (e = dmalloc(&b.dIdx, cap)) || (e = dmalloc(&b.dPerm, cap)) || (e = dmalloc(&b.dReq, kMaxRuns)) ||
+      (e = dmalloc(&b.dBlk, nblk)))
     return e;
   if ((e = rocprim::radix_sort_pairs(nullptr, b.sortBytes, b.dKey, b.dKey2, b.dIdx, b.dPerm, cap, 0, 40, b.s))) return e;
   return hipMalloc(&b.sortTmp, b.sortBytes + 16);
 }
 
-// Takes pending chunks, earliest deadline first, that fit one batch.  Caller holds mu.
-void take_requests(mlod_server* S, BatchSlot& b) {
-  b.reqs.clear();
-  uint32_t slots = 0;
-  while (!S->pending.empty() && b.reqs.size() < kMaxReqPerBatch && slots + S->pending.top().m <= S->cap) {
-    slots += S->pending.top().m;
-    b.reqs.push_back(S->pending.top());
-    S->pending.pop();
-  }
+void free_pool(mlod_server* S) {
+  void* d[] = {S->dQ, S->dQn, S->dPidx};
+  for (void* p : d)
+    if (p) (void)hipFree(p);
+  S->dQ = S->dQn = nullptr;
+  S->dPidx = nullptr;
 }
 
-// Stages and launches the requests in b.reqs (worker thread, without the mutex).
-int launch_batch(mlod_server* S, BatchSlot& b) {
-  // consecutive chunks of one request share a descriptor, so the request's queries are sorted
-  // and walked together
-  uint32_t off = 0, nReq = 0;
-  b.chunkOff.resize(b.reqs.size());
-  for (size_t r = 0; r < b.reqs.size(); ++r) {
-    const SReq& c = b.reqs[r];
-    const Request& q = *c.req;
-    b.chunkOff[r] = off;
-    std::memcpy(b.hIn + 3 * (size_t)off, q.in + 3 * (size_t)c.off, (size_t)c.m * 3 * sizeof(float));
-    const bool extend = r > 0 && b.reqs[r - 1].req == c.req && b.reqs[r - 1].off + b.reqs[r - 1].m == c.off;
-    off += c.m;
-    if (extend) {
-      b.hReq[nReq - 1].count += c.m;
+hipError_t alloc_pool(mlod_server* S, uint32_t blocks) {
+  hipError_t e;
+  const size_t slots = (size_t)blocks * kBatchWG;
+  if ((e = dmalloc(&S->dQ, slots)) || (e = dmalloc(&S->dQn, slots)) || (e = dmalloc(&S->dPidx, slots))) return e;
+  S->poolBlocks = S->freeBlocks = blocks;
+  S->freeRanges.clear();
+  S->freeRanges[0] = blocks;
+  return hipSuccess;
+}
+
+// First fit; false when no free range is long enough.
+bool pool_alloc(mlod_server* S, uint32_t n, uint32_t& blk0) {
+  for (auto it = S->freeRanges.begin(); it != S->freeRanges.end(); ++it) {
+    if (it->second < n) continue;
+    blk0 = it->first;
+    const uint32_t rest = it->second - n;
+    S->freeRanges.erase(it);
+    if (rest) S->freeRanges[blk0 + n] = rest;
+    S->freeBlocks -= n;
+    return true;
+  }
+  return false;
+}
+
+void pool_free(mlod_server* S, uint32_t blk0, uint32_t n) {
+  auto it = S->freeRanges.emplace(blk0, n).first;
+  auto nx = std::next(it);
+  if (nx != S->freeRanges.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    S->freeRanges.erase(nx);
+  }
+  if (it != S->freeRanges.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      S->freeRanges.erase(it);
+    }
+  }
+  S->freeBlocks += n;
+}
+
+double ms_since(const mlod_server* S, clk::time_point t) {
+  return std::chrono::duration<double, std::milli>(t - S->t0).count();
+}
+
+// Scheduling order: class (interactive first), deadline, submission.
+struct Urgency {
+  int cls;
+  double deadline;
+  uint64_t seq;
+  bool operator<(const Urgency& o) const {
+    if (cls != o.cls) return cls < o.cls;
+    if (deadline != o.deadline) return deadline < o.deadline;
+    return seq < o.seq;
+  }
+};
+inline Urgency urgency(const Request* q, uint64_t seq) { return Urgency{q->interactive ? 0 : 1, q->deadline, seq}; }
+
+// A candidate of the next launch: a resident run waiting in the pool, or a queued chunk.
+struct Cand {
+  Urgency u;
+  Run* run;
+  Chunk ch;
+};
+
+// Fills st.runs with the runs of the stream's next launch: candidates of class cls, most urgent
+// first, at most S->cap queries; queued chunks become new runs (consecutive chunks of a request
+// one run) when pool blocks are free for them.  Returns the queries picked; the sequence numbers
+// of the chunks placed go to `placed`.
+uint32_t take_class(mlod_server* S, Stream& st, const std::vector<Cand>& cands, int cls, std::vector<uint64_t>& placed) {
+  const uint32_t steps = cls == 0 ? kInteractiveSteps : kBulkSteps;
+  // bulk chunks leave one launch's worth of pool blocks to interactive ones
+  const uint32_t reserve = cls == 0 ? 0u : max_launch_blocks(S->cap);
+  uint32_t used = 0;
+  std::vector<Chunk> taken;
+  for (const Cand& c : cands) {
+    const uint32_t m = c.run ? c.run->m : c.ch.m;
+    if (c.u.cls != cls || used + m > S->cap || st.runs.size() >= kMaxRuns) continue;
+    used += m;
+    if (c.run) {
+      c.run->steps = std::min(steps, c.run->stepsLeft);
+      st.runs.push_back(c.run);
       continue;
     }
+    taken.push_back(c.ch);
+    Run* last = st.runs.empty() ? nullptr : st.runs.back();
+    if (last && last->fresh && last->req == c.ch.req && last->off + last->m == c.ch.off) {
+      last->m += m;
+      continue;
+    }
+    Run* r = new Run{c.ch.req, c.ch.off, m, c.ch.seq, 0, 0, c.ch.req->nIter, 0, -1, true, false};
+    r->steps = std::min(steps, r->stepsLeft);
+    st.runs.push_back(r);
+  }
+  // pool blocks for the new runs; a run without them stays queued as its chunks
+  std::vector<Run*> keep;
+  uint32_t picked = 0;
+  for (Run* r : st.runs) {
+    if (r->fresh) {
+      const uint32_t nb = blocks_of(r->m);
+      uint32_t b0 = 0;
+      if (S->freeBlocks < nb + reserve || !pool_alloc(S, nb, b0)) {
+        delete r;
+        continue;
+      }
+      r->blk0 = b0;
+      r->nblk = nb;
+      S->runs.push_back(r);
+      for (const Chunk& c : taken)
+        if (c.req == r->req && c.off >= r->off && c.off < r->off + r->m) placed.push_back(c.seq);
+    }
+    keep.push_back(r);
+    picked += r->m;
+  }
+  st.runs.swap(keep);
+  return picked;
+}
+
+// Picks the next launch of stream st; returns its queries (0: nothing to launch).  Interactive
+// work goes first, except that bulk work waiting for more than kBulkMaxWaitMs since the last
+// bulk launch gets the launch, so a stream of interactive requests cannot starve it.
+uint32_t pick_runs(mlod_server* S, Stream& st, bool& interactive) {
+  const double now = ms_since(S, clk::now());
+  std::vector<Cand> cands;
+  for (Run* r : S->runs)
+    if (r->stream < 0) cands.push_back(Cand{urgency(r->req, r->seq), r, Chunk{}});
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (ChunkQueue* Q : {&S->qInt, &S->qBulk}) {
+      uint32_t got = 0;
+      while (!Q->empty() && got < S->cap) {
+        const Chunk c = Q->top();
+        Q->pop();
+        got += c.m;
+        cands.push_back(Cand{urgency(c.req, c.seq), nullptr, c});
+      }
+    }
+  }
+  if (cands.empty()) return 0;
+  std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.u < b.u; });
+  int cls = cands[0].u.cls;
+  if (cls == 0 && cands.back().u.cls == 1 && now - S->lastBulkMs > kBulkMaxWaitMs) cls = 1;
+  std::vector<uint64_t> placed;
+  uint32_t picked = take_class(S, st, cands, cls, placed);
+  if (picked == 0 && cands.front().u.cls != cands.back().u.cls) {
+    // nothing of that class fits the pool now: the other class
+    cls = 1 - cls;
+    picked = take_class(S, st, cands, cls, placed);
+  }
+  interactive = cls == 0;
+  if (picked && !interactive) S->lastBulkMs = now;
+  std::sort(placed.begin(), placed.end());
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (const Cand& c : cands)
+    if (!c.run && !std::binary_search(placed.begin(), placed.end(), c.ch.seq))
+      (c.ch.req->interactive ? S->qInt : S->qBulk).push(c.ch);
+  return picked;
+}
+
+// Stages the new runs of st.runs and launches the steps of all of them (worker thread, without
+// the mutex).  Runs taking their last steps get result rows in the stream's host staging.
+int launch_step(mlod_server* S, Stream& st, bool interactive) {
+  const int sid = (int)(&st - S->streams.data());
+  // new runs first: their staging keys index them
+  std::stable_partition(st.runs.begin(), st.runs.end(), [](const Run* r) { return r->fresh; });
+  uint32_t nNew = 0, staged = 0, nBlk = 0, outRows = 0;
+  const size_t nRuns = st.runs.size();
+  st.outOff.assign(nRuns, kNoOut);
+  st.blk0.assign(nRuns, 0);
+  st.seen.assign(nRuns, 0);
+  const auto now = clk::now();
+  for (size_t k = 0; k < nRuns; ++k) {
+    Run* r = st.runs[k];
+    r->stream = sid;
+    Request& q = *r->req;
     const mlod_tree* t = S->trees[q.tree];
-    ReqDesc& d = b.hReq[nReq++];
+    ReqDesc& d = st.hReq[k];
     d.tv = t->view();
     d.w = make_wcoef(q.kernel);
     d.cutoff = q.cutoff;
     d.stepMax = q.stepMax < 0.f ? t->radius : q.stepMax;
     d.minDepth = q.minDepth;
-    d.nIter = q.nIter;
-    d.slot0 = b.chunkOff[r];
-    d.count = c.m;
+    d.nIter = r->steps;
+    d.count = r->m;
+    d.pad0 = r->blk0 * kBatchWG;
+    d.slot0 = 0;
+    d.outOff = kNoOut;
     for (int a = 0; a < 3; ++a) {
       const float ext = t->BB[a] - t->bb[a];
       d.org[a] = t->bb[a];
       d.scale[a] = ext > 0.f ? 1024.f / ext : 0.f;
     }
+    if (r->fresh) {
+      d.slot0 = staged;
+      std::memcpy(st.hIn + 3 * (size_t)staged, q.in + 3 * (size_t)r->off, (size_t)r->m * 3 * sizeof(float));
+      staged += r->m;
+      ++nNew;
+      if (!q.started) {
+        q.started = true;
+        q.launched = now;
+      }
+    }
+    st.blk0[k] = nBlk;
+    for (uint32_t b = 0; b < r->nblk; ++b) st.hBlk[nBlk++] = make_uint2((uint32_t)k, r->blk0 + b);
+    if (r->steps == r->stepsLeft) {
+      d.outOff = st.outOff[k] = outRows;
+      outRows += r->m;
+    }
   }
-  uint32_t blk = 0;
-  for (uint32_t r = 0; r < nReq; ++r) {
-    ReqDesc& d = b.hReq[r];
-    d.pad0 = blk * kBatchWG;
-    const uint32_t nb = (d.count + kBatchWG - 1) / kBatchWG;
-    for (uint32_t i = 0; i < nb; ++i) b.hBlockReq[blk + i] = r;
-    blk += nb;
+  hipError_t e = hipMemcpyAsync(st.dReq, st.hReq, nRuns * sizeof(ReqDesc), hipMemcpyHostToDevice, st.s);
+  if (!e) e = hipMemcpyAsync(st.dBlk, st.hBlk, nBlk * sizeof(uint2), hipMemcpyHostToDevice, st.s);
+  if (!e && staged) {
+    const uint32_t grid = (staged + BS - 1) / BS;
+    int runBits = 1;
+    while ((1u << runBits) < nNew) ++runBits;
+    e = hipMemcpyAsync(st.dIn, st.hIn, (size_t)staged * 3 * sizeof(float), hipMemcpyHostToDevice, st.s);
+    if (!e) {
+      k_batch_keys<<<grid, BS, 0, st.s>>>(st.dIn, staged, st.dReq, nNew, st.dKey, st.dIdx);
+      e = hipGetLastError();
+    }
+    if (!e)
+      e = rocprim::radix_sort_pairs(st.sortTmp, st.sortBytes, st.dKey, st.dKey2, st.dIdx, st.dPerm, staged, 0,
+                                    30 + runBits, st.s);
+    if (!e) {
+      k_pool_gather<<<grid, BS, 0, st.s>>>(st.dIn, st.dKey2, st.dPerm, staged, st.dReq, S->dQ, S->dQn, S->dPidx);
+      e = hipGetLastError();
+    }
   }
-  const uint32_t slots = off;
-  b.slots = slots;
-  int reqBits = 1;
-  while ((1u << reqBits) < nReq) ++reqBits;
-  hipError_t e = hipMemcpyAsync(b.dIn, b.hIn, (size_t)slots * 3 * sizeof(float), hipMemcpyHostToDevice, b.s);
-  if (!e) e = hipMemcpyAsync(b.dReq, b.hReq, nReq * sizeof(ReqDesc), hipMemcpyHostToDevice, b.s);
-  if (!e) e = hipMemcpyAsync(b.dBlockReq, b.hBlockReq, blk * sizeof(uint32_t), hipMemcpyHostToDevice, b.s);
-  const uint32_t grid = (slots + BS - 1) / BS;
-  if (!e) {
-    k_batch_keys<<<grid, BS, 0, b.s>>>(b.dIn, slots, b.dReq, nReq, b.dKey, b.dIdx);
-    e = hipGetLastError();
-  }
-  if (!e) e = rocprim::radix_sort_pairs(b.sortTmp, b.sortBytes, b.dKey, b.dKey2, b.dIdx, b.dPerm, slots, 0, 30 + reqBits, b.s);
-  if (!e) {
-    k_batch_gather<<<grid, BS, 0, b.s>>>(b.dIn, b.dKey2, b.dPerm, slots, b.dReq, b.dQ, b.dQn);
-    e = hipGetLastError();
-  }
-  if (e) return hip_fail(e, "serving batch", __FILE__, __LINE__);
-  if (int r = launch_project_batch(b.dReq, b.dBlockReq, b.dQ, b.dQn, 0, blk, S->semantics, S->precision, b.s)) return r;
-  k_batch_scatter<<<grid, BS, 0, b.s>>>(b.dQ, b.dQn, b.dKey2, b.dPerm, b.dReq, slots, S->cap, b.dOut);
-  e = hipGetLastError();
-  if (!e) e = hipMemcpyAsync(b.hOut, b.dOut, (size_t)slots * 3 * sizeof(float), hipMemcpyDeviceToHost, b.s);
-  if (!e)
-    e = hipMemcpyAsync(b.hOut + 3 * (size_t)S->cap, b.dOut + 3 * (size_t)S->cap, (size_t)slots * 3 * sizeof(float),
-                       hipMemcpyDeviceToHost, b.s);
-  if (!e) e = hipEventRecord(b.ev, b.s);
-  if (e) return hip_fail(e, "serving batch", __FILE__, __LINE__);
-  b.launched = clk::now();
-  for (const SReq& c : b.reqs)
-    if (c.off == 0) c.req->launched = b.launched;
+  if (e) return hip_fail(e, "serving launch", __FILE__, __LINE__);
+  for (Run* r : st.runs) r->fresh = false;
+  st.seq += 1;
+  const BatchOut bo{S->dPidx, st.dOut, S->cap, st.dFlag, st.seq};
+  if (int rc = launch_project_batch(st.dReq, st.dBlk, nBlk, S->dQ, S->dQn, bo, interactive ? 1u : 0u, S->semantics,
+                                    S->precision, st.s))
+    return rc;
+  e = hipEventRecord(st.ev, st.s);
+  if (e) return hip_fail(e, "serving launch", __FILE__, __LINE__);
   return MLOD_OK;
 }
 
-// Copies a finished batch's results to the callers (worker thread, without the mutex) and
-// returns the completion records of the requests it finished.
-void complete_batch(mlod_server* S, BatchSlot& b, std::vector<mlod_server_done>& out, std::vector<Request*>& finished) {
-  for (size_t r = 0; r < b.reqs.size(); ++r) {
-    const SReq& c = b.reqs[r];
-    const Request& q = *c.req;
-    const uint32_t off = b.chunkOff[r];
-    if (q.outP) std::memcpy(q.outP + 3 * (size_t)c.off, b.hOut + 3 * (size_t)off, (size_t)c.m * 3 * sizeof(float));
-    if (q.outN)
-      std::memcpy(q.outN + 3 * (size_t)c.off, b.hOut + 3 * ((size_t)S->cap + off), (size_t)c.m * 3 * sizeof(float));
-  }
-  const auto now = clk::now();
-  for (const SReq& c : b.reqs) {
-    Request& q = *c.req;
-    if (--q.chunksLeft) continue;
+// Copies the results of run k of st's launch (all its workgroups flagged done) to its caller,
+// frees its pool blocks, and collects its request if that is complete.  Returns its queries.
+uint64_t hand_back(mlod_server* S, Stream& st, size_t k, clk::time_point now, std::vector<mlod_server_done>& out,
+                   std::vector<Request*>& finished) {
+  Run* r = st.runs[k];
+  Request& q = *r->req;
+  const size_t o = st.outOff[k];
+  if (q.outP) std::memcpy(q.outP + 3 * (size_t)r->off, st.hOut + 3 * o, (size_t)r->m * 3 * sizeof(float));
+  if (q.outN)
+    std::memcpy(q.outN + 3 * (size_t)r->off, st.hOut + 3 * ((size_t)S->cap + o), (size_t)r->m * 3 * sizeof(float));
+  pool_free(S, r->blk0, r->nblk);
+  S->runs.erase(std::find(S->runs.begin(), S->runs.end(), r));
+  r->handed = true;
+  q.left -= r->m;
+  if (q.left == 0) {
     mlod_server_done d;
     d.ticket = q.ticket;
     d.latency_ms = std::chrono::duration<double, std::milli>(now - q.submitted).count();
@@ -348,6 +545,41 @@ void complete_batch(mlod_server* S, BatchSlot& b, std::vector<mlod_server_done>&
     out.push_back(d);
     finished.push_back(&q);
   }
+  return r->m;
+}
+
+// Hands back the runs of st's launch whose workgroups have all flagged done.
+uint64_t poll_runs(mlod_server* S, Stream& st, std::vector<mlod_server_done>& out, std::vector<Request*>& finished) {
+  uint64_t handed = 0;
+  const volatile uint32_t* flag = st.hFlag;
+  for (size_t k = 0; k < st.runs.size(); ++k) {
+    Run* r = st.runs[k];
+    if (st.outOff[k] == kNoOut || r->handed) continue;
+    while (st.seen[k] < r->nblk && flag[st.blk0[k] + st.seen[k]] == st.seq) ++st.seen[k];
+    if (st.seen[k] < r->nblk) continue;
+    std::atomic_thread_fence(std::memory_order_acquire);
+    handed += hand_back(S, st, k, clk::now(), out, finished);
+  }
+  return handed;
+}
+
+// A stream's launch has completed: hands back the runs that took their last steps and are not
+// handed back yet; the other runs wait in the pool again (worker thread, without the mutex).
+uint64_t complete_step(mlod_server* S, Stream& st, std::vector<mlod_server_done>& out,
+                       std::vector<Request*>& finished) {
+  const auto now = clk::now();
+  uint64_t handed = 0;
+  for (size_t k = 0; k < st.runs.size(); ++k) {
+    Run* r = st.runs[k];
+    r->stepsLeft -= r->steps;
+    r->stream = -1;
+    if (st.outOff[k] == kNoOut) continue;
+    if (!r->handed) handed += hand_back(S, st, k, now, out, finished);
+  }
+  for (Run* r : st.runs)
+    if (r->handed) delete r;
+  st.runs.clear();
+  return handed;
 }
 
 void set_error(mlod_server* S, int code, const std::string& msg) {
@@ -366,58 +598,62 @@ void worker_main(mlod_server* S) {
   while (true) {
     bool progressed = false, anyBusy = false;
     // completions: copy out, then publish
-    for (BatchSlot& b : S->slots) {
-      if (!b.busy) continue;
-      const hipError_t q = hipEventQuery(b.ev);
-      if (q == hipSuccess) {
-        fin.clear();
-        finished.clear();
-        complete_batch(S, b, fin, finished);
-        b.reqs.clear();
-        b.busy = false;
-        for (Request* q : finished) delete q;
+    for (Stream& st : S->streams) {
+      if (!st.busy) continue;
+      const hipError_t q = hipEventQuery(st.ev);
+      fin.clear();
+      finished.clear();
+      uint64_t handed = 0;
+      if (q == hipErrorNotReady) handed = poll_runs(S, st, fin, finished);
+      if (q == hipSuccess || handed) {
+        if (q == hipSuccess) {
+          handed += complete_step(S, st, fin, finished);
+          st.busy = false;
+        } else {
+          anyBusy = true;
+        }
+        for (Request* r : finished) delete r;
         {
           std::lock_guard<std::mutex> lk(S->mu);
           S->done.insert(S->done.end(), fin.begin(), fin.end());
           S->requests += fin.size();
+          S->queries += handed;
         }
-        S->cvDone.notify_all();
+        if (!fin.empty()) S->cvDone.notify_all();
         progressed = true;
       } else if (q != hipErrorNotReady) {
-        set_error(S, MLOD_ERR_HIP, std::string("serving batch failed: ") + hipGetErrorString(q));
+        set_error(S, MLOD_ERR_HIP, std::string("serving launch failed: ") + hipGetErrorString(q));
         return;
       } else {
         anyBusy = true;
       }
     }
-    // launches into free slots
-    for (BatchSlot& b : S->slots) {
-      if (b.busy) continue;
-      {
-        std::lock_guard<std::mutex> lk(S->mu);
-        take_requests(S, b);
-      }
-      if (b.reqs.empty()) break;
-      const int r = launch_batch(S, b);
+    // next launches on idle streams
+    for (Stream& st : S->streams) {
+      if (st.busy) continue;
+      bool interactive = false;
+      if (pick_runs(S, st, interactive) == 0) break;
+      const int r = launch_step(S, st, interactive);
       if (r) {
         set_error(S, r, g_last_error());
         return;
       }
-      b.busy = true;
+      st.busy = true;
       anyBusy = true;
       progressed = true;
       std::lock_guard<std::mutex> lk(S->mu);
-      S->batches += 1;
-      S->queries += b.slots;
+      S->launches += 1;
     }
     std::unique_lock<std::mutex> lk(S->mu);
-    if (S->stop && S->pending.empty() && !anyBusy) return;
+    const bool queued = !S->qInt.empty() || !S->qBulk.empty();
+    if (S->stop && !queued && !anyBusy && S->runs.empty()) return;
     if (progressed) continue;
     if (anyBusy) {
       lk.unlock();
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      std::this_thread::yield();
     } else {
-      S->cv.wait(lk, [S] { return S->stop || !S->pending.empty(); });
+      S->cv.wait_for(lk, std::chrono::milliseconds(1),
+                     [S] { return S->stop || !S->qInt.empty() || !S->qBulk.empty(); });
     }
   }
 }
@@ -433,9 +669,9 @@ int mlod_server_create(uint32_t max_batch_queries, uint32_t chunk_queries, uint3
                        int32_t precision, mlod_server** out) {
   if (!out) return fail(MLOD_ERR_INVALID, "out is NULL");
   *out = nullptr;
-  if (max_batch_queries < kBatchWG || inflight == 0 || inflight > 8)
-    return fail(MLOD_ERR_INVALID, "bad batch capacity or inflight");
-  if (chunk_queries > max_batch_queries) return fail(MLOD_ERR_INVALID, "chunk larger than a batch");
+  if (max_batch_queries < kBatchWG || max_batch_queries > (1u << 24) || inflight == 0 || inflight > 8)
+    return fail(MLOD_ERR_INVALID, "bad launch capacity or inflight");
+  if (chunk_queries > max_batch_queries) return fail(MLOD_ERR_INVALID, "chunk larger than a launch");
   if (precision != MLOD_PREC_FP32 && precision != MLOD_PREC_FP32C)
     return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
   if (semantics != MLOD_SEM_REF_GPU && semantics != MLOD_SEM_REF_CPU) return fail(MLOD_ERR_INVALID, "bad semantics");
@@ -445,14 +681,14 @@ int mlod_server_create(uint32_t max_batch_queries, uint32_t chunk_queries, uint3
   S->semantics = semantics;
   S->precision = precision;
   (void)hipGetDevice(&S->device);
-  S->slots.resize(inflight);
-  for (BatchSlot& b : S->slots) {
-    const hipError_t e = alloc_slot(b, S->cap);
-    if (e) {
-      for (BatchSlot& c : S->slots) free_slot(c);
-      delete S;
-      return hip_fail(e, "server buffers", __FILE__, __LINE__);
-    }
+  S->streams.resize(inflight);
+  hipError_t e = alloc_pool(S, kPoolLaunches * inflight * max_launch_blocks(S->cap));
+  for (size_t i = 0; !e && i < S->streams.size(); ++i) e = alloc_stream(S->streams[i], S->cap);
+  if (e) {
+    for (Stream& c : S->streams) free_stream(c);
+    free_pool(S);
+    delete S;
+    return hip_fail(e, "server buffers", __FILE__, __LINE__);
   }
   S->worker = std::thread(worker_main, S);
   *out = S;
@@ -467,15 +703,19 @@ int mlod_server_destroy(mlod_server* S) {
   }
   S->cv.notify_all();
   if (S->worker.joinable()) S->worker.join();
+  for (Stream& st : S->streams) free_stream(st);
   // requests left behind by a failed worker
   std::vector<Request*> left;
-  for (BatchSlot& b : S->slots)
-    for (const SReq& c : b.reqs) left.push_back(c.req);
-  for (; !S->pending.empty(); S->pending.pop()) left.push_back(S->pending.top().req);
+  for (Run* r : S->runs) {
+    left.push_back(r->req);
+    delete r;
+  }
+  for (; !S->qInt.empty(); S->qInt.pop()) left.push_back(S->qInt.top().req);
+  for (; !S->qBulk.empty(); S->qBulk.pop()) left.push_back(S->qBulk.top().req);
   std::sort(left.begin(), left.end());
   left.erase(std::unique(left.begin(), left.end()), left.end());
   for (Request* q : left) delete q;
-  for (BatchSlot& b : S->slots) free_slot(b);
+  free_pool(S);
   delete S;
   return MLOD_OK;
 }
@@ -511,6 +751,10 @@ int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, f
   r->outP = out_pts;
   r->outN = out_nmls;
   r->m = m;
+  r->left = m;
+  r->started = false;
+  const double pts = (double)m * std::max<uint32_t>(n_iter, 1u);
+  r->interactive = pts <= kInteractivePts;
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (tree_id >= S->trees.size()) {
@@ -520,12 +764,25 @@ int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, f
     r->ticket = S->nextTicket++;
     r->submitted = clk::now();
     r->launched = r->submitted;
-    const double now_ms = std::chrono::duration<double, std::milli>(r->submitted - S->t0).count();
-    const double deadline = now_ms + (double)m * std::max<uint32_t>(n_iter, 1u) / kNominalPtsPerMs;
-    r->chunksLeft = (m + S->chunk - 1) / S->chunk;
-    for (uint32_t off = 0; off < m; off += S->chunk)
-      S->pending.push(SReq{r, off, std::min(S->chunk, m - off), deadline, S->nextSeq++});
     *ticket = r->ticket;
+    if (n_iter == 0) {
+      // no step: the queries come back unchanged with zero normals, as from a standalone projection
+      if (out_pts) std::memcpy(out_pts, queries, (size_t)m * 3 * sizeof(float));
+      if (out_nmls) std::memset(out_nmls, 0, (size_t)m * 3 * sizeof(float));
+      mlod_server_done d;
+      d.ticket = r->ticket;
+      d.latency_ms = d.queue_ms = 0.0;
+      d.completed_s = std::chrono::duration<double>(clk::now() - S->t0).count();
+      S->done.push_back(d);
+      S->requests += 1;
+      S->queries += m;
+      delete r;
+      S->cvDone.notify_all();
+      return MLOD_OK;
+    }
+    r->deadline = ms_since(S, r->submitted) + pts / kNominalPtsPerMs;
+    ChunkQueue& Q = r->interactive ? S->qInt : S->qBulk;
+    for (uint32_t off = 0; off < m; off += S->chunk) Q.push(Chunk{r, off, std::min(S->chunk, m - off), S->nextSeq++});
   }
   S->cv.notify_one();
   return MLOD_OK;
@@ -547,10 +804,10 @@ int mlod_server_poll(mlod_server* S, mlod_server_done* out, uint32_t cap, uint32
 int mlod_server_stats(mlod_server* S, uint64_t* out4) {
   if (!S || !out4) return fail(MLOD_ERR_INVALID, "NULL argument");
   std::lock_guard<std::mutex> lk(S->mu);
-  out4[0] = S->batches;
+  out4[0] = S->launches;
   out4[1] = S->requests;
   out4[2] = S->queries;
-  out4[3] = S->pending.size();
+  out4[3] = S->qInt.size() + S->qBulk.size();
   return MLOD_OK;
 }
 

@@ -29,9 +29,10 @@ __all__ = ["Server", "serving_workload", "run_load"]
 
 
 class Server:
-    """A serving instance: resident trees, a worker thread, up to ``inflight`` batches of at
-    most ``max_batch`` queries on the GPU at once; requests are cut into chunks of ``chunk``
-    queries (0: max_batch / 4)."""
+    """A serving instance: resident trees, a worker thread and ``inflight`` streams, each with
+    one step launch of at most ``max_batch`` queries on the GPU at a time; requests are cut into
+    chunks of ``chunk`` queries (0: max_batch / 4) that stay in a device pool between steps.
+    ``stats()['batches']`` counts step launches."""
 
     def __init__(self, max_batch=131072, inflight=3, semantics="ref_gpu", precision="fp32c", chunk=0):
         sem = {"ref_gpu": 0, "ref_cpu": 1}[semantics]

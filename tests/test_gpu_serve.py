@@ -66,6 +66,7 @@ def test_alone_equals_mixed_equals_standalone(trees, semantics, precision):
     srv_a = Server(max_batch=8192, inflight=2, semantics=semantics, precision=precision)
     tids = [srv_a.add_tree(t) for t in tr]
     alone = _serve(srv_a, tids, reqs, together=False)
+    # alone, an interactive request takes all its steps in one launch
     assert srv_a.stats()["batches"] == len(reqs)
     srv_a.close()
     srv_b = Server(max_batch=8192, inflight=3, semantics=semantics, precision=precision, chunk=1024)
@@ -74,7 +75,7 @@ def test_alone_equals_mixed_equals_standalone(trees, semantics, precision):
     st = srv_b.stats()
     srv_b.close()
     assert st["requests"] == len(reqs) and st["queries"] == sum(len(r["q"]) for r in reqs)
-    assert st["batches"] < len(reqs)  # requests were packed together
+    assert st["batches"] < len(reqs)  # requests shared launches
     assert max(len(r["q"]) for r in reqs) > 1024  # and some were cut into chunks
     for r, (pa, na), (pm, nm) in zip(reqs, alone, mixed):
         assert np.array_equal(pa, pm, equal_nan=True) and np.array_equal(na, nm, equal_nan=True)
@@ -102,3 +103,31 @@ def test_server_rejects_bad_requests(trees):
         Server(precision="fp64")
     with pytest.raises(MlodError):
         Server(max_batch=4096, chunk=8192)  # chunk larger than a batch
+
+
+def test_bulk_and_interactive_mix(trees):
+    """Bulk requests (more than 2M projected points) advance two steps per launch, wait in the
+    pool while interactive ones overtake them, and still give the standalone bits; a request of
+    zero steps returns its queries with zero normals."""
+    clouds, tr = trees
+    rng = np.random.default_rng(21)
+    reqs = []
+    for c, m, it in [(0, 60_000, 35), (2, 45_000, 47), (1, 3_000, 3), (0, 700, 10), (2, 9_000, 1), (1, 20_000, 5)]:
+        P = clouds[c][0]
+        rad = tr[c].radius()
+        q = (P[rng.integers(0, len(P), m)] + rng.uniform(-0.02, 0.02, (m, 3)) * rad).astype(np.float32)
+        k = Kernel(type=2, sigmas=[float(np.float32(0.01 * 2.0 ** i) * np.float32(rad)) for i in range(4)])
+        reqs.append(dict(tree=c, q=q, kernel=k, cutoff=2.0, min_depth=0, n_iter=it))
+    srv = Server(max_batch=16384, inflight=3, chunk=4096)
+    tids = [srv.add_tree(t) for t in tr]
+    got = _serve(srv, tids, reqs, together=True)
+    z = srv.submit(tids[0], reqs[3]["q"], reqs[3]["kernel"], 2.0, 0, 0)
+    zero = srv.poll(timeout=5.0)
+    st = srv.stats()
+    srv.close()
+    assert st["requests"] == len(reqs) + 1 and st["pending"] == 0
+    assert zero[0][0] == z and np.array_equal(zero[0][4], reqs[3]["q"]) and not zero[0][5].any()
+    for r, (p, n) in zip(reqs, got):
+        ps, ns, _ = gpu_project(tr[r["tree"]], r["q"], r["kernel"], cutoff=r["cutoff"], min_depth=r["min_depth"],
+                                semantics="ref_gpu", precision="fp32c", n_iter=r["n_iter"])
+        assert np.array_equal(p, ps, equal_nan=True) and np.array_equal(n, ns, equal_nan=True)
