@@ -58,8 +58,8 @@ struct KParams {
   float c;           // constant_shift
   float cp;          // max(c, 1e-4)
   float e;           // exponent
-  float a[MLOD_MAX_SIGMAS];
-  float b[MLOD_MAX_SIGMAS];
+  float a[MLOD_MAX_SIGMAS];     // FP32: -log2(e) / (2 s^2)
+  float b[MLOD_MAX_SIGMAS];     // FP32: 1 / s^3
   double ad[MLOD_MAX_SIGMAS];  // FP64: -1 / (2 s^2)
   double bd[MLOD_MAX_SIGMAS];  // FP64: 1 / s^3
 };

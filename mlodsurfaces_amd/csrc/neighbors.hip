@@ -91,8 +91,8 @@ struct SumD {
 template <int KT>
 __device__ __forceinline__ float weight32(const KParams& k, float d2) {
   if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
-    float w = 0.f;
-    for (int s = 0; s < k.ns; ++s) w += __builtin_amdgcn_exp2f(fmaf(d2, k.a[s], k.b[s]));
+    float w = k.b[0] * __builtin_amdgcn_exp2f(d2 * k.a[0]);
+    for (int s = 1; s < k.ns; ++s) w = fmaf(k.b[s], __builtin_amdgcn_exp2f(d2 * k.a[s]), w);
     return w == 0.f ? k.c : w;
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN) {
     return __builtin_amdgcn_exp2f(d2 * k.a[0]) + k.c;

@@ -50,14 +50,21 @@ __device__ __forceinline__ float weight32(const KP& k, float d2) {
     if (k.type == MLOD_KERNEL_GAUSSIAN) return weight32<MLOD_KERNEL_GAUSSIAN>(k, d2);
     return weight32<MLOD_KERNEL_SINGULAR>(k, d2);
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
-    float w = 0.f;
+    // sum over s of sigma_s^-3 * 2^(a_s d2), in order of s
+    float w = k.b[0] * __builtin_amdgcn_exp2f(d2 * k.a[0]);
     if (k.ns <= 8) {
-      // statically indexed, so the coefficients are loaded once into scalar registers
+      // statically indexed, so the coefficients sit in scalar registers; the wave-uniform exit
+      // is one scalar compare and branch per term (the empty asm keeps the compiler from
+      // hoisting the compares into lane masks), so no term beyond ns is evaluated
+      int ns = k.ns;
 #pragma unroll
-      for (int s = 0; s < 8; ++s)
-        if (s < k.ns) w += __builtin_amdgcn_exp2f(fmaf(d2, k.a[s], k.b[s]));
+      for (int s = 1; s < 8; ++s) {
+        asm volatile("" : "+s"(ns));
+        if (s >= ns) break;
+        w = fmaf(k.b[s], __builtin_amdgcn_exp2f(d2 * k.a[s]), w);
+      }
     } else {
-      for (int s = 0; s < k.ns; ++s) w += __builtin_amdgcn_exp2f(fmaf(d2, k.a[s], k.b[s]));
+      for (int s = 1; s < k.ns; ++s) w = fmaf(k.b[s], __builtin_amdgcn_exp2f(d2 * k.a[s]), w);
     }
     return w == 0.f ? k.c : w;  // an all-underflowed sum falls back to constant_shift (Kernel.cpp:69-73)
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN) {
@@ -786,7 +793,7 @@ KParams make_kparams(const mlod_kernel& k) {
     for (int s = 0; s < k.n_sigmas && s < MLOD_MAX_SIGMAS; ++s) {
       const double sg = k.sigmas[s];
       p.a[s] = (float)(-log2e / (2.0 * sg * sg));
-      p.b[s] = (float)(-3.0 * std::log2(sg));
+      p.b[s] = (float)(1.0 / (sg * sg * sg));
       p.ad[s] = -1.0 / (2.0 * sg * sg);
       p.bd[s] = 1.0 / (sg * sg * sg);
     }
