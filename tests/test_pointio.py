@@ -1,0 +1,133 @@
+"""Point-set files, the normalised frame and .params settings (mlodsurfaces_amd.pointio), and
+the oracle on the reference's own sample models: the oracle reproduces the reference's
+projectCPU bit for bit on Face (8 sigmas) and a half-igea subset (15 sigmas)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from mlodsurfaces_amd import pointio
+
+DATA = "/root/reference/data"
+REAL = ["real_Face.npz", "real_half-igea.npz"]
+
+
+def test_params_fields_and_sigmas():
+    """scene::loadParameters field order: the decoded settings are the paper's (kernel type 2 =
+    multi-Gaussian, cutoff, min depth, iterations), and the generated sigmas are
+    beginningSigma * factor^i times the radius (Kernel::updateSigmas), to float32 rounding."""
+    f = pointio.read_params(load_golden("real_Face.npz")["params"].tobytes())
+    assert (f.kernel_type, f.n_computed_sigmas, f.max_iterations, f.min_depth, f.method) == (2, 8, 10, 1, 1)
+    assert f.computed_sigma and not f.cut and abs(f.cutoff - 1.4) < 1e-6 and abs(f.factor - 2.0) < 1e-7
+    assert (f.n_points, f.max_upsamples, f.max_depth) == (100000, 100, 6)
+    h = pointio.read_params(load_golden("real_half-igea.npz")["params"].tobytes())
+    assert (h.kernel_type, h.n_computed_sigmas, h.min_depth) == (2, 15, 0) and abs(h.cutoff - 2.0) < 1e-7
+    for p in (f, h):
+        R = 0.8125
+        k = p.kernel(R)
+        assert k.type == 2 and len(k.sigmas) == p.n_computed_sigmas
+        closed = np.array([p.beginning_sigma * p.factor ** i * R for i in range(p.n_computed_sigmas)])
+        assert np.allclose(k.sigmas, closed, rtol=4e-6 * p.n_computed_sigmas, atol=0)
+        assert abs(k.sigma - p.sigma * R) <= 1e-6 * abs(p.sigma * R)
+    with pytest.raises(ValueError):
+        pointio.read_params(b"\0" * 100)
+
+
+@pytest.mark.skipif(not os.path.isdir(DATA), reason="the reference's data folder is not present")
+def test_reader_on_reference_files():
+    """The reader parses the reference's own sample files: the fixtures hold what it read, and
+    every .params file decodes to a known kernel type with positive sigmas."""
+    for name in ("Face", "half-igea"):
+        P, N = pointio.read_points(os.path.join(DATA, name + ".ply"))
+        g = load_golden(f"real_{name}.npz")
+        if name == "Face":
+            assert np.array_equal(P, g["points"]) and np.array_equal(N, g["normals"])
+        assert len(P) == int(open(os.path.join(DATA, name + ".ply"), "rb").read(200).split(b"element vertex ")[1].split()[0])
+        assert np.allclose(np.linalg.norm(N, axis=1), 1.0, atol=1e-3)
+    for f in sorted(os.listdir(DATA)):
+        if f.endswith(".params"):
+            p = pointio.read_params(os.path.join(DATA, f))
+            assert p.kernel_type in (0, 1, 2) and 0 < p.cutoff < 10 and p.max_iterations >= 1
+            assert all(s > 0 for s in p.kernel(1.0).sigmas)
+
+
+def test_ply_pn_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    P = rng.standard_normal((1000, 3)).astype(np.float32)
+    N = rng.standard_normal((1000, 3)).astype(np.float32)
+    for ext in ("ply", "pn"):
+        path = str(tmp_path / f"c.{ext}")
+        pointio.write_points(path, P, N)
+        P2, N2 = pointio.read_points(path)
+        assert np.array_equal(P, P2) and np.array_equal(N, N2)
+    # header and record layout of Pn::save: binary little-endian, x y z nx ny nz float records
+    raw = open(str(tmp_path / "c.ply"), "rb").read()
+    head, body = raw.split(b"end_header\n", 1)
+    assert head.split(b"\n")[:3] == [b"ply", b"format binary_little_endian 1.0", b"element vertex 1000"]
+    assert np.array_equal(np.frombuffer(body, "<f4").reshape(1000, 6), np.concatenate([P, N], 1))
+    # positions without normals, and a frame mapped back on write
+    fr = pointio.Frame(np.array([1.0, -2.0, 0.5], np.float32), 3.0)
+    pointio.write_points(str(tmp_path / "d.ply"), P, None, fr)
+    P3, N3 = pointio.read_points(str(tmp_path / "d.ply"))
+    assert N3 is None and np.array_equal(P3, (P * np.float32(3.0) + fr.offset).astype(np.float32))
+
+
+def test_ply_ascii_and_extra_elements(tmp_path):
+    """ASCII PLY with properties in another order, an extra property and a face list after the
+    vertices; a binary PLY with a uchar colour and a face element."""
+    txt = ("ply\nformat ascii 1.0\ncomment test\nelement vertex 3\nproperty float nx\nproperty float ny\n"
+           "property float nz\nproperty double x\nproperty double y\nproperty double z\nproperty float quality\n"
+           "element face 1\nproperty list uchar int vertex_indices\nend_header\n"
+           "0 0 1 1.5 2.5 3.5 7\n0 1 0 -1 -2 -3 8\n1 0 0 0.25 0.5 0.75 9\n3 0 1 2\n")
+    p = tmp_path / "a.ply"
+    p.write_text(txt)
+    P, N = pointio.read_points(str(p))
+    assert np.array_equal(P, np.array([[1.5, 2.5, 3.5], [-1, -2, -3], [0.25, 0.5, 0.75]], np.float32))
+    assert np.array_equal(N, np.array([[0, 0, 1], [0, 1, 0], [1, 0, 0]], np.float32))
+    head = ("ply\nformat binary_little_endian 1.0\nelement vertex 2\nproperty float x\nproperty float y\n"
+            "property float z\nproperty uchar red\nproperty float nx\nproperty float ny\nproperty float nz\n"
+            "element face 1\nproperty list uchar int vertex_indices\nend_header\n").encode()
+    dt = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("r", "u1"), ("nx", "<f4"), ("ny", "<f4"), ("nz", "<f4")])
+    v = np.array([(1, 2, 3, 200, 0, 0, 1), (4, 5, 6, 100, 1, 0, 0)], dt)
+    (tmp_path / "b.ply").write_bytes(head + v.tobytes() + bytes([3]) + np.array([0, 1, 1], "<i4").tobytes())
+    P, N = pointio.read_points(str(tmp_path / "b.ply"))
+    assert np.array_equal(P, [[1, 2, 3], [4, 5, 6]]) and np.array_equal(N, [[0, 0, 1], [1, 0, 0]])
+
+
+def test_normalize_frame():
+    """Pn::normalize: offset = the float32 running sum / n (p0 + p1 + ... in order), the padded
+    box's largest |coordinate| is the norme, and the frame maps the points back."""
+    g = load_golden("real_Face.npz")
+    P = g["points"]
+    Pn, fr = pointio.normalize(P)
+    acc = P[0].copy()
+    for i in range(1, len(P)):  # the reference's loop, one float32 add per point
+        acc = (acc + P[i]).astype(np.float32)
+    assert np.array_equal(fr.offset, (acc / np.float32(len(P))).astype(np.float32))
+    assert np.allclose(fr.offset, P.astype(np.float64).mean(0), atol=1e-5)
+    assert np.abs(Pn).max() < 1.0 and abs(np.abs(Pn).max() - 1.0) < 0.02
+    back = Pn.astype(np.float64) * fr.norme + fr.offset
+    assert np.abs(back - P).max() <= 4 * np.finfo(np.float32).eps * np.abs(P).max()
+
+
+@pytest.mark.parametrize("name", REAL)
+def test_oracle_matches_reference_on_real_models(oracle_built, name):
+    """The oracle's FP32 restatement of projectCPU gives the reference's bits on the reference's
+    own models with their own settings (after 1 and 3 steps); the FP64 oracle is within the
+    1e-5 x diag gate of it."""
+    g = load_golden(name)
+    prm = pointio.read_params(g["params"].tobytes())
+    Pn, _ = pointio.normalize(g["points"])
+    o = oracle_built.OracleTree(Pn, g["normals"])
+    assert o.ties == int(g["ties"]) == 0 and o.num_nodes == int(g["ref_nodes"])
+    K = prm.kernel(float(g["radius"]))
+    k = oracle_built.make_kernel(type=K.type, sigma=K.sigma, exponent=K.exponent, constant_shift=K.constant_shift,
+                                 sigmas=K.sigmas)
+    par = oracle_built.make_params(cutoff=prm.cutoff, min_depth=prm.min_depth, semantics="ref_cpu")
+    diag = 2 * float(g["radius"])
+    for it in (1, 3):
+        op, on = o.project(g["queries"], k, par, n_iter=it, precision=0)
+        assert np.array_equal(op.astype(np.float32), g[f"p{it}"]) and np.array_equal(on.astype(np.float32), g[f"n{it}"])
+        op64, _ = o.project(g["queries"], k, par, n_iter=it, precision=1)
+        assert (np.linalg.norm(op64 - g[f"p{it}"], axis=1) / diag).max() < 1e-5
