@@ -130,6 +130,63 @@ def adaptive_measure(A, ds, w, depth, semantics, precision):
             "ms": dt * 1e3}
 
 
+def real_models_measure(A):
+    """The reference's own sample models (Face; a 24k-point subset of half-igea) with their own
+    .params settings: each model's points projected maxNbIterations steps (FP32C, REF_GPU) and
+    timed with HIP events; error against the FP64 form; and, with REF_CPU semantics, the distance
+    to the reference's projectCPU results stored with the model (4096 queries, 3 steps)."""
+    from mlodsurfaces_amd import hiprt, pointio
+    out = {}
+    for name in ("Face", "half-igea"):
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", f"real_{name}.npz")
+        if not os.path.exists(path):
+            continue
+        with np.load(path, allow_pickle=False) as z:
+            g = {k: z[k] for k in z.files}
+        prm = pointio.read_params(g["params"].tobytes())
+        Pn, _ = pointio.normalize(g["points"])
+        tree = A.APSSOctree().build(Pn, g["normals"])
+        K = prm.kernel(tree.radius())
+        diag = 2.0 * tree.radius()
+        steps = max(prm.max_iterations, 1)
+        res = {}
+        for prec in ("fp32c", "fp64"):
+            ap = A.Fastapss(tree, K)
+            ap.setScalingProtectionSphere(prm.cutoff)
+            ap.setMinDepth(prm.min_depth)
+            ap.set_precision(prec)
+            ap.copyPointsToGPU(len(Pn), Pn)
+            ap.project(len(Pn), None, None, 1, K)  # warm-up step
+            ap.copyPointsToGPU(len(Pn), Pn)
+            s = hiprt.Stream()
+            e0, e1 = hiprt.Event(), hiprt.Event()
+            hiprt.device_synchronize()
+            e0.record(s)
+            ap.project(len(Pn), None, None, steps, K, stream=s.ptr)
+            e1.record(s)
+            e1.synchronize()
+            op, _ = ap.download(len(Pn))
+            res[prec] = (op, e0.elapsed_ms(e1))
+        err = np.linalg.norm(res["fp32c"][0].astype(np.float64) - res["fp64"][0], axis=1) / diag
+        ap = A.Fastapss(tree, K)
+        ap.setScalingProtectionSphere(prm.cutoff)
+        ap.setMinDepth(prm.min_depth)
+        ap.set_semantics("ref_cpu")
+        ap.set_precision("fp32c")
+        ap.copyPointsToGPU(len(g["queries"]), g["queries"])
+        rp, _ = ap.project(len(g["queries"]), np.zeros_like(g["queries"]), np.zeros_like(g["queries"]), 3, K)
+        rerr = np.linalg.norm(rp.astype(np.float64) - g["p3"], axis=1) / diag
+        ms = res["fp32c"][1]
+        out[name] = {"points": len(Pn), "sigmas": len(K.sigmas), "cutoff": prm.cutoff, "min_depth": prm.min_depth,
+                     "steps": steps, "ms": ms, "pts_per_s": len(Pn) * steps / (ms * 1e-3),
+                     "fp64_ms": res["fp64"][1], "fp32c_vs_fp64_max": float(err.max()),
+                     "ref_cpu_vs_reference_projectCPU_max": float(rerr.max())}
+        log(f"real model {name}: {len(Pn)} points, {len(K.sigmas)} sigmas, {steps} steps in {ms:.2f} ms "
+            f"({out[name]['pts_per_s'] / 1e6:.1f} M pts/s; FP64 {res['fp64'][1]:.2f} ms), FP32C vs FP64 max "
+            f"{err.max():.2e}, REF_CPU FP32C vs the reference's projectCPU max {rerr.max():.2e} x diag")
+    return out
+
+
 def lib_handle():
     from mlodsurfaces_amd import _lib
     return _lib.lib()
@@ -346,6 +403,9 @@ def main():
         adaptive = adaptive_measure(A, ds, w, args.adaptive_depth, args.semantics, args.precision)
     del tree, dP, dN, dQ
     hiprt.device_synchronize()
+    real = None
+    if args.workload == "superquadric8m" and args.scale == 1.0:
+        real = real_models_measure(A)
     serving = None
     if not args.no_serve and args.workload == "superquadric8m" and args.scale == 1.0:
         try:
@@ -387,6 +447,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "adaptive_octree": adaptive,
+        "real_models": real,
     }
     print(json.dumps(out), flush=True)
 

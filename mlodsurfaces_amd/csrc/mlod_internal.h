@@ -51,15 +51,15 @@ struct TreeView {
 //   multi-Gaussian  w = sum_s exp2(d2 * a[s] + b[s])   (a = -log2(e)/(2 s^2), b = -3 log2 s)
 //   Gaussian        w = exp2(d2 * a[0]) + c
 //   rational        w = (c' / (d2 + c))^e,  c' = max(c, 1e-4)
-struct KParams {
+struct alignas(16) KParams {
+  float a[MLOD_MAX_SIGMAS];     // FP32: -log2(e) / (2 s^2)  (first, 16-byte aligned: read four at a time)
+  float b[MLOD_MAX_SIGMAS];     // FP32: 1 / s^3
   int32_t type;
   int32_t ns;
   int32_t eint;      // integer exponent (1..16) or -1
   float c;           // constant_shift
   float cp;          // max(c, 1e-4)
   float e;           // exponent
-  float a[MLOD_MAX_SIGMAS];     // FP32: -log2(e) / (2 s^2)
-  float b[MLOD_MAX_SIGMAS];     // FP32: 1 / s^3
   double ad[MLOD_MAX_SIGMAS];  // FP64: -1 / (2 s^2)
   double bd[MLOD_MAX_SIGMAS];  // FP64: 1 / s^3
 };

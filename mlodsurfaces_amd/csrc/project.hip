@@ -52,7 +52,8 @@ __device__ __forceinline__ float weight32(const KP& k, float d2) {
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
     // sum over s of sigma_s^-3 * 2^(a_s d2), in order of s
     float w = k.b[0] * __builtin_amdgcn_exp2f(d2 * k.a[0]);
-    if (k.ns <= 8) {
+    constexpr int kCap = sizeof(KP::a) / sizeof(float);  // 8 for the serving coefficients
+    if (kCap <= 8 || k.ns <= 8) {
       // statically indexed, so the coefficients sit in scalar registers; the wave-uniform exit
       // is one scalar compare and branch per term (the empty asm keeps the compiler from
       // hoisting the compares into lane masks), so no term beyond ns is evaluated
@@ -63,8 +64,27 @@ __device__ __forceinline__ float weight32(const KP& k, float d2) {
         if (s >= ns) break;
         w = fmaf(k.b[s], __builtin_amdgcn_exp2f(d2 * k.a[s]), w);
       }
-    } else {
-      for (int s = 1; s < k.ns; ++s) w = fmaf(k.b[s], __builtin_amdgcn_exp2f(d2 * k.a[s]), w);
+    } else if constexpr (kCap > 8) {
+      // more sigmas (the sample models' settings use up to 30): the first eight from scalar
+      // registers, the rest four at a time, each group's coefficients fetched by two 16-byte
+      // scalar loads (the table is 16-byte aligned) rather than one dependent load per term
+#pragma unroll
+      for (int s = 1; s < 8; ++s) w = fmaf(k.b[s], __builtin_amdgcn_exp2f(d2 * k.a[s]), w);
+      int ns = k.ns;
+      for (int s0 = 8; s0 < ns; s0 += 4) {
+        const float4 A4 = *reinterpret_cast<const float4*>(k.a + s0);
+        const float4 B4 = *reinterpret_cast<const float4*>(k.b + s0);
+        w = fmaf(B4.x, __builtin_amdgcn_exp2f(d2 * A4.x), w);
+        asm volatile("" : "+s"(ns));
+        if (s0 + 1 >= ns) break;
+        w = fmaf(B4.y, __builtin_amdgcn_exp2f(d2 * A4.y), w);
+        asm volatile("" : "+s"(ns));
+        if (s0 + 2 >= ns) break;
+        w = fmaf(B4.z, __builtin_amdgcn_exp2f(d2 * A4.z), w);
+        asm volatile("" : "+s"(ns));
+        if (s0 + 3 >= ns) break;
+        w = fmaf(B4.w, __builtin_amdgcn_exp2f(d2 * A4.w), w);
+      }
     }
     return w == 0.f ? k.c : w;  // an all-underflowed sum falls back to constant_shift (Kernel.cpp:69-73)
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN) {
