@@ -220,6 +220,41 @@ int mlod_generator_cells(mlod_generator* g, int32_t which, int32_t* out_ids4, vo
  * until the next call on g.  These are the query points of the reference's upsampling. */
 int mlod_generator_centres(mlod_generator* g, float* out_xyz, const float** d_xyz, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Dual contouring of the adaptive octree: scene::computeDualContouring (scene.cpp:1255-1400),
+ * the reference's mesh output from the projection.  Stages:
+ *   mlod_dc_build    uniformize(max_depth, cells, ..., cut) (upsample.cpp:262-272): re-root,
+ *                    grade, regularize up and siblings, optional cut; cells are n x (depth, x, y, z)
+ *                    int32 (host), usually mlod_generator_cells(g, 1, ...);
+ *   corner values    mlod_dc_project_corners: every leaf corner projected one step on tree t with
+ *                    its current kernel and settings, value (c - p')·n', NaN where |n'|^2 < 0.5
+ *                    (scene.cpp:1346-1387); or mlod_dc_set_corner_values (host, one per corner in
+ *                    mlod_dc_corners order);
+ *   mlod_dc_extract  dual_contour::extract (dual_contouring.h:59-87) over the minimal edges:
+ *                    a vertex per cell of a sign-changing minimal edge (mean of its edges' zero
+ *                    crossings, the cell centre when there is none), two triangles per edge.
+ * Positions are in the frame of the projection ([-1, 1]^3 of the adaptive octree: new2bbox);
+ * unit positions in the uniformized octree's [0, 1]^3.  Vertices are numbered in node order and
+ * triangles in (leaf, edge) order; the reference numbers them in its traversal's order. */
+typedef struct mlod_dc mlod_dc;
+int mlod_dc_create(mlod_dc** out);
+int mlod_dc_destroy(mlod_dc* dc);
+int mlod_dc_build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t max_depth, int32_t cut, void* stream);
+/* [nodes, leaves, corners, max depth, diag, origin x, origin y, origin z, extracted] */
+int mlod_dc_info(const mlod_dc* dc, double* out9);
+/* the uniformized octree in node_comparator order: ids4 (n x 4) and leaf flags (either may be NULL) */
+int mlod_dc_nodes(const mlod_dc* dc, int32_t* ids4, uint8_t* is_leaf, void* stream);
+/* the distinct leaf corners: integer positions at the finest depth (n x 3) and positions in the
+ * projection frame (n x 3 floats); either may be NULL */
+int mlod_dc_corners(const mlod_dc* dc, int32_t* xyz3, float* pos, void* stream);
+int mlod_dc_project_corners(mlod_dc* dc, mlod_tree* t, void* stream);
+int mlod_dc_set_corner_values(mlod_dc* dc, const float* f, void* stream);
+int mlod_dc_corner_values(const mlod_dc* dc, float* f, void* stream);
+int mlod_dc_extract(mlod_dc* dc, uint32_t* n_verts, uint32_t* n_tris, void* stream);
+/* vertices (projection frame and unit frame, n x 3 floats), the octree node of each vertex, and
+ * triangles (n x 3 vertex indices); any may be NULL */
+int mlod_dc_mesh(const mlod_dc* dc, float* verts, float* unit_verts, uint32_t* vert_node, uint32_t* tris, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,17 @@ SIGNATURES = {
     "mlod_generator_info": (C.c_int, [_vp, _u64]),
     "mlod_generator_cells": (C.c_int, [_vp, C.c_int32, _vp, _vp]),
     "mlod_generator_centres": (C.c_int, [_vp, _vp, C.POINTER(_vp), _vp]),
+    "mlod_dc_create": (C.c_int, [C.POINTER(_vp)]),
+    "mlod_dc_destroy": (C.c_int, [_vp]),
+    "mlod_dc_build": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_int32, _vp]),
+    "mlod_dc_info": (C.c_int, [_vp, _vp]),
+    "mlod_dc_nodes": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mlod_dc_corners": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "mlod_dc_project_corners": (C.c_int, [_vp, _vp, _vp]),
+    "mlod_dc_set_corner_values": (C.c_int, [_vp, _vp, _vp]),
+    "mlod_dc_corner_values": (C.c_int, [_vp, _vp, _vp]),
+    "mlod_dc_extract": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _vp]),
+    "mlod_dc_mesh": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

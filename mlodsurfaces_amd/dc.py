@@ -1,0 +1,126 @@
+"""Dual contouring of the adaptive octree on the GPU (libmlod.so's mlod_dc_*): the reference's
+mesh output from the projection (scene::computeDualContouring, sources/scene/scene.cpp:1255-1460).
+
+``DualContouring(cells, max_depth, cut)`` uniformizes the cells (upsample.cpp:262-272);
+``project_corners(tree)`` takes the corner field from the tree's projection (or
+``set_corner_values`` from the caller); ``extract()`` returns the mesh.  ``surface_mesh`` runs the
+reference's sequence on a model: adaptive octree, uniformize, corner field, extraction, and the
+vertices projected three steps onto the surface (scene.cpp:1417-1452, without its smoothing and
+subdivision passes)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+
+__all__ = ["DualContouring", "surface_mesh"]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class DualContouring:
+    def __init__(self, cells, max_depth=0, cut=False, stream=None):
+        ids = np.ascontiguousarray(cells, np.int32).reshape(-1, 4)
+        h = C.c_void_p()
+        check(lib().mlod_dc_create(C.byref(h)))
+        self.h = h
+        self._stream = stream
+        check(lib().mlod_dc_build(self.h, _p(ids), len(ids), int(max_depth), int(bool(cut)), stream))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mlod_dc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        o = np.zeros(9, np.float64)
+        check(lib().mlod_dc_info(self.h, _p(o)))
+        return dict(nodes=int(o[0]), leaves=int(o[1]), corners=int(o[2]), max_depth=int(o[3]), diag=float(o[4]),
+                    origin=o[5:8].astype(np.float32), extracted=bool(o[8]))
+
+    def nodes(self):
+        n = self.info()["nodes"]
+        ids = np.zeros((n, 4), np.int32)
+        leaf = np.zeros(n, np.uint8)
+        check(lib().mlod_dc_nodes(self.h, _p(ids), _p(leaf), self._stream))
+        return ids, leaf.astype(bool)
+
+    def corners(self):
+        """(integer corner positions at the finest depth, positions in the projection frame)"""
+        n = self.info()["corners"]
+        xyz = np.zeros((n, 3), np.int32)
+        pos = np.zeros((n, 3), np.float32)
+        check(lib().mlod_dc_corners(self.h, _p(xyz), _p(pos), self._stream))
+        return xyz, pos
+
+    def project_corners(self, tree):
+        """Corner values from one projection step of every corner on `tree` (its current kernel,
+        cutoff, min depth, mode, semantics and form)."""
+        check(lib().mlod_dc_project_corners(self.h, tree.handle, self._stream))
+
+    def set_corner_values(self, f):
+        f = np.ascontiguousarray(f, np.float32)
+        if len(f) != self.info()["corners"]:
+            raise ValueError("one value per corner")
+        check(lib().mlod_dc_set_corner_values(self.h, _p(f), self._stream))
+
+    def corner_values(self):
+        f = np.zeros(self.info()["corners"], np.float32)
+        check(lib().mlod_dc_corner_values(self.h, _p(f), self._stream))
+        return f
+
+    def extract(self):
+        """(vertices in the projection frame, triangles, node of each vertex, unit-frame vertices)"""
+        nv, nt = C.c_uint32(), C.c_uint32()
+        check(lib().mlod_dc_extract(self.h, C.byref(nv), C.byref(nt), self._stream))
+        v = np.zeros((nv.value, 3), np.float32)
+        u = np.zeros((nv.value, 3), np.float32)
+        vn = np.zeros(nv.value, np.uint32)
+        t = np.zeros((nt.value, 3), np.uint32)
+        check(lib().mlod_dc_mesh(self.h, _p(v), _p(u), _p(vn), _p(t), self._stream))
+        return v, t, vn, u
+
+
+def surface_mesh(apss, depth, cut=False, reproject_steps=3, log=None):
+    """The reference's mesh of the surface for an APSS plugin `apss` (Fastapss or GlobalAPSS with
+    its kernel and settings installed): OctreeGenerator.generate(depth) with its crossed cells,
+    uniformize, corner field by projection, dual contouring, then the vertices projected
+    `reproject_steps` steps (scene.cpp:1417-1433).  Returns (vertices, normals, triangles, info)."""
+    import time
+    from .apss import OctreeGenerator
+    t0 = time.perf_counter()
+    gen = OctreeGenerator(apss, apss.kernel if hasattr(apss, "kernel") else None)
+    total = []
+    gen.generate(depth, total)
+    cells = total[0]
+    t1 = time.perf_counter()
+    dc = DualContouring(cells, depth, cut)
+    inf = dc.info()
+    t2 = time.perf_counter()
+    apss._sync_params()
+    dc.project_corners(apss.m_apssOctree)
+    t3 = time.perf_counter()
+    V, F, VN, U = dc.extract()
+    t4 = time.perf_counter()
+    N = np.zeros_like(V)
+    if reproject_steps and len(V):
+        V, N = apss.project_points(V, reproject_steps)
+    t5 = time.perf_counter()
+    info = dict(cells=len(cells), nodes=inf["nodes"], leaves=inf["leaves"], corners=inf["corners"], vertices=len(V),
+                triangles=len(F), ms_generate=(t1 - t0) * 1e3, ms_uniformize=(t2 - t1) * 1e3,
+                ms_corners=(t3 - t2) * 1e3, ms_extract=(t4 - t3) * 1e3, ms_reproject=(t5 - t4) * 1e3)
+    if log:
+        log(f"surface mesh depth {depth}: {info['cells']} crossed cells -> {info['nodes']} nodes, {info['leaves']} "
+            f"leaves, {info['corners']} corners -> {info['vertices']} vertices, {info['triangles']} triangles")
+    dc.close()
+    return V, N, F, info

@@ -100,6 +100,16 @@ def oracle_lib():
         L.orc_gen_from_points.argtypes = [_vp, _f, C.c_uint32, C.c_uint32, _f]
         L.orc_gen_info.argtypes = [_vp, _u64]
         L.orc_gen_ids.argtypes = [_vp, C.c_int32, _i]
+        L.orc_dc_build.restype = _vp
+        L.orc_dc_build.argtypes = [_i, C.c_uint32, C.c_uint32, C.c_int32]
+        L.orc_dc_free.argtypes = [_vp]
+        L.orc_dc_num_nodes.restype = C.c_uint32
+        L.orc_dc_num_nodes.argtypes = [_vp]
+        L.orc_dc_frame.argtypes = [_vp, _f, _i]
+        L.orc_dc_nodes.argtypes = [_vp, _i, C.POINTER(C.c_uint8)]
+        L.orc_dc_set_corners.argtypes = [_vp, _i, _f, C.c_uint32]
+        L.orc_dc_extract.argtypes = [_vp, _u, _u]
+        L.orc_dc_mesh.argtypes = [_vp, _f, _u, _u]
         L.orc_cell_center.argtypes = [_i, _f, _f]
         L.orc_cell_ancestor.argtypes = [_i, C.c_int32, _i]
         _orc = L
@@ -353,3 +363,45 @@ def oracle_cell_ancestor(id4, depth):
     out = np.zeros(4, np.int32)
     L.orc_cell_ancestor(_ptr(i, C.c_int32), int(depth), _ptr(out, C.c_int32))
     return out
+
+
+class OracleDC:
+    """Dual contouring of the oracle (dc_oracle.cpp): uniformize the given octree cells (ids4:
+    depth, x, y, z), set corner values at max-depth integer corner positions, extract."""
+
+    def __init__(self, ids4, max_depth=0, cut=False):
+        self.L = oracle_lib()
+        ids = np.ascontiguousarray(ids4, np.int32).reshape(-1, 4)
+        self.h = self.L.orc_dc_build(_ptr(ids, C.c_int32), len(ids), int(max_depth), int(bool(cut)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_dc_free(self.h)
+            self.h = None
+
+    def nodes(self):
+        n = self.L.orc_dc_num_nodes(self.h)
+        ids = np.zeros((n, 4), np.int32)
+        leaf = np.zeros(n, np.uint8)
+        self.L.orc_dc_nodes(self.h, _ptr(ids, C.c_int32), _ptr(leaf, C.c_uint8))
+        return ids, leaf.astype(bool)
+
+    def frame(self):
+        o = np.zeros(4, np.float32)
+        d = C.c_int32()
+        self.L.orc_dc_frame(self.h, _ptr(o, C.c_float), C.byref(d))
+        return float(o[0]), o[1:].copy(), int(d.value)
+
+    def set_corners(self, xyz, f):
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        f = np.ascontiguousarray(f, np.float32)
+        self.L.orc_dc_set_corners(self.h, _ptr(xyz, C.c_int32), _ptr(f, C.c_float), len(f))
+
+    def extract(self):
+        nv, nt = C.c_uint32(), C.c_uint32()
+        self.L.orc_dc_extract(self.h, C.byref(nv), C.byref(nt))
+        v = np.zeros((nv.value, 3), np.float32)
+        vn = np.zeros(nv.value, np.uint32)
+        t = np.zeros((nt.value, 3), np.uint32)
+        self.L.orc_dc_mesh(self.h, _ptr(v, C.c_float), _ptr(vn, C.c_uint32), _ptr(t, C.c_uint32))
+        return v, vn, t

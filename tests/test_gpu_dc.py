@@ -1,0 +1,121 @@
+"""Dual contouring on the GPU (mlod_dc_*, mlodsurfaces_amd.dc) against the oracle's restatement
+(oracle/dc_oracle.cpp): the same uniformized octree, corners and mesh — every triangle, as the
+octree cells of its three vertices, and every vertex position bit for bit — for an analytic corner
+field and for the corner field of the projection on a point cloud; the projected-field mesh is
+closed and its vertices, projected three steps, lie on the surface."""
+from collections import Counter
+
+import numpy as np
+import pytest
+
+from mlodsurfaces_amd import apss as A
+from mlodsurfaces_amd import datasets as ds
+from mlodsurfaces_amd.dc import DualContouring, surface_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def near_sphere_cells(d=6, c=0.8, r=0.5, band=0.03):
+    g = (np.arange(-2 ** d, 2 * 2 ** d) + 0.5) / 2 ** d
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    near = np.abs(np.linalg.norm(P - c, axis=1) - r) < band
+    return np.concatenate([np.full((near.sum(), 1), d), np.floor(P[near] * 2 ** d).astype(int)], 1).astype(np.int32)
+
+
+def inside(ids):
+    w = 1 << ids[:, 0]
+    return np.all((ids[:, 1:] >= 0) & (ids[:, 1:] < w[:, None]), 1)
+
+
+def compare_meshes(gpu, orc, nodes, onodes):
+    """Triangles as the cells of their vertices, and the vertex of every cell (the GPU drops the
+    cells outside the root, so node indices differ)."""
+    V, F, VN, U = gpu
+    ov, ovn, ot = orc
+    key = lambda arr, n: tuple(int(q) for q in arr[n])  # noqa: E731
+    g_tris = Counter(tuple(key(nodes, VN[i]) for i in t) for t in F)
+    o_tris = Counter(tuple(key(onodes, ovn[i]) for i in t) for t in ot)
+    assert g_tris == o_tris
+    g_pos = {key(nodes, n): tuple(u) for n, u in zip(VN, U)}
+    o_pos = {key(onodes, n): tuple(u) for n, u in zip(ovn, ov)}
+    assert g_pos.keys() == o_pos.keys()
+    assert all(np.array_equal(np.float32(g_pos[k]), np.float32(o_pos[k])) for k in g_pos)
+
+
+def test_uniformize_and_analytic_field_match_oracle(oracle_built):
+    ids = near_sphere_cells()
+    dc = DualContouring(ids)
+    inf = dc.info()
+    nodes, leaf = dc.nodes()
+    o = oracle_built.OracleDC(ids)
+    on, ol = o.nodes()
+    keep = inside(on)
+    assert np.array_equal(nodes, on[keep]) and np.array_equal(leaf, ol[keep])
+    diag, origin, D = o.frame()
+    assert inf["diag"] == diag and np.array_equal(inf["origin"], origin) and inf["max_depth"] == D
+    xyz, pos = dc.corners()
+    L = nodes[leaf]
+    sc = (1 << (D - L[:, 0]))[:, None]
+    expect = np.unique(np.concatenate([(L[:, 1:] + np.array([c & 1, (c >> 1) & 1, (c >> 2) & 1])) * sc
+                                       for c in range(8)]), axis=0)
+    assert {tuple(r) for r in xyz} == {tuple(r) for r in expect} and len(xyz) == len(expect)
+    # new2bbox: position = (2 diag) (c / 2^D) + (2 origin - 1), exactly
+    m00, m03 = np.float32(2 * diag), (np.float32(2) * origin - np.float32(1)).astype(np.float32)
+    assert np.array_equal(pos, (m00 * (xyz.astype(np.float32) / np.float32(2.0 ** D)) + m03).astype(np.float32))
+    # an analytic field in the projection frame, the same values on both sides: the sphere the
+    # cells were taken around (cell position p at depth d lies at -1 + 2 (p + 1/2) / 2^d)
+    f = (np.linalg.norm(pos.astype(np.float64) - 0.6, axis=1) - 1.0).astype(np.float32)
+    f[::97] = np.nan  # failed projections
+    dc.set_corner_values(f)
+    gpu = dc.extract()
+    o.set_corners(xyz, f)
+    orc = o.extract()
+    assert len(gpu[1]) > 1000
+    compare_meshes(gpu, orc, nodes, on)
+    V, F, VN, U = gpu
+    assert np.array_equal(V, (m00 * U + m03).astype(np.float32))
+
+
+def test_projection_field_pipeline(oracle_built):
+    p, n = ds.sphere(40000, seed=5, noise=0.0)
+    p = (p * 0.6 + np.array([0.05, -0.1, 0.1])).astype(np.float32)
+    tree = A.APSSOctree().build(p, n)
+    R = tree.radius()
+    k = A.Kernel(type=2, sigmas=[float(np.float32(0.01 * 2 ** i) * np.float32(R)) for i in range(4)])
+    ap = A.Fastapss(tree, k)
+    ap.setScalingProtectionSphere(2.0)
+    ap.set_precision("fp32c")
+    gen = A.OctreeGenerator(ap, k)
+    total = []
+    gen.generate(6, total)
+    cells = total[0]
+    dc = DualContouring(cells, 6)
+    ap._sync_params()
+    dc.project_corners(tree)
+    f = dc.corner_values()
+    xyz, pos = dc.corners()
+    # the corner field is the signed distance to the sphere of radius 0.6 where the projection works
+    ok = np.isfinite(f)
+    assert ok.mean() > 0.95
+    sd = np.linalg.norm(pos[ok].astype(np.float64) - [0.05, -0.1, 0.1], axis=1) - 0.6
+    near = np.abs(sd) < 0.1
+    assert np.abs(f[ok][near] - sd[near]).max() < 2e-3
+    gpu = dc.extract()
+    nodes, leaf = dc.nodes()
+    o = oracle_built.OracleDC(cells)
+    on, ol = o.nodes()
+    assert np.array_equal(nodes, on[inside(on)])
+    o.set_corners(xyz, f)
+    compare_meshes(gpu, o.extract(), nodes, on)
+    V, F, VN, U = gpu
+    E = Counter()
+    for a, b, c in F:
+        for e in ((a, b), (b, c), (c, a)):
+            E[e] += 1
+    assert max(E.values()) == 1 and set(Counter((min(a, b), max(a, b)) for a, b in E).values()) == {2}
+    # the whole pipeline: vertices projected three steps lie on the surface
+    V3, N3, F3, info = surface_mesh(ap, 6)
+    assert info["triangles"] == len(F) and info["vertices"] == len(V)
+    r = np.linalg.norm(V3.astype(np.float64) - [0.05, -0.1, 0.1], axis=1)
+    assert np.abs(r - 0.6).max() < 1e-3
