@@ -255,6 +255,16 @@ int mlod_dc_extract(mlod_dc* dc, uint32_t* n_verts, uint32_t* n_tris, void* stre
  * triangles (n x 3 vertex indices); any may be NULL */
 int mlod_dc_mesh(const mlod_dc* dc, float* verts, float* unit_verts, uint32_t* vert_node, uint32_t* tris, void* stream);
 
+/* Mesh passes of the surface output (host arrays in and out), the reference's results bit for bit:
+ * mesh::smoothMesh (upsample.h:55-76): every vertex moved to the mean of the other vertices of
+ * its triangles (NaN for a vertex no triangle uses); mesh::subdivideMesh (upsample.h:78-133): a
+ * vertex at the middle of every edge, numbered in order of first use, each triangle split in
+ * four (out_tris: 4 nt triangles; out_verts must hold nv + 3 nt vertices; *out_nv receives the
+ * count). */
+int mlod_mesh_smooth(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out_verts, void* stream);
+int mlod_mesh_subdivide(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out_verts,
+                        uint32_t out_cap, uint32_t* out_tris, uint32_t* out_nv, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

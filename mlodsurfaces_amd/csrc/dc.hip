@@ -385,6 +385,101 @@ __global__ __launch_bounds__(BS) void k_to_frame(const float* __restrict__ u, ui
   out[3 * (size_t)i + 2] = __fadd_rn(__fmul_rn(m00.z, u[3 * (size_t)i + 2]), m03.z);
 }
 
+// ----------------------------------------------------------------------------- mesh passes
+// mesh::smoothMesh and mesh::subdivideMesh (sources/scene/upsample.h:55-133), in the reference's
+// float operations and orders: a stable sort by vertex (smoothing) or by edge (subdivision) keeps
+// the triangle order in which the sequential code accumulates and numbers, so the results are
+// those of the reference bit for bit.
+
+// encounter e = 3 t + k of triangle t: smoothing key = its k-th vertex; subdivision key = its
+// k-th edge (a, b), (c, b), (c, a) as (min, max)
+__global__ __launch_bounds__(BS) void k_mesh_keys(const uint32_t* __restrict__ tri, uint32_t nt, int edges,
+                                                  uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const uint32_t e = blockIdx.x * BS + threadIdx.x;
+  if (e >= 3 * nt) return;
+  const uint32_t t = e / 3, k = e % 3;
+  const uint32_t a = tri[3 * (size_t)t], b = tri[3 * (size_t)t + 1], c = tri[3 * (size_t)t + 2];
+  if (edges) {
+    const uint32_t u = k == 0 ? a : c, v = k == 0 ? b : (k == 1 ? b : a);
+    key[e] = ((uint64_t)min(u, v) << 32) | (uint64_t)max(u, v);
+  } else {
+    key[e] = k == 0 ? a : (k == 1 ? b : c);
+  }
+  val[e] = e;
+}
+
+// smoothMesh: newPos[v] += (the other two vertices of each of v's triangles, in triangle order),
+// valence 2 per triangle, then newPos / valence (NaN for an unreferenced vertex, as 0 / 0)
+__global__ __launch_bounds__(BS) void k_smooth(const float* __restrict__ P, uint32_t nv, const uint32_t* __restrict__ tri,
+                                               const uint64_t* __restrict__ skey, const uint32_t* __restrict__ senc,
+                                               uint32_t ne, float* __restrict__ out) {
+  const uint32_t v = blockIdx.x * BS + threadIdx.x;
+  if (v >= nv) return;
+  uint32_t lo = 0, hi = ne;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (skey[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  float s[3] = {0.f, 0.f, 0.f}, val = 0.f;
+  for (uint32_t i = lo; i < ne && skey[i] == v; ++i) {
+    const uint32_t e = senc[i], t = e / 3, k = e % 3;
+    const uint32_t a = tri[3 * (size_t)t], b = tri[3 * (size_t)t + 1], c = tri[3 * (size_t)t + 2];
+    // a: P[b] + P[c]; b: P[a] + P[c]; c: P[b] + P[a]
+    const uint32_t x = k == 0 ? b : (k == 1 ? a : b), y = k == 0 ? c : (k == 1 ? c : a);
+    for (int q = 0; q < 3; ++q) s[q] = __fadd_rn(s[q], __fadd_rn(P[3 * (size_t)x + q], P[3 * (size_t)y + q]));
+    val = __fadd_rn(val, 2.0f);
+  }
+  for (int q = 0; q < 3; ++q) out[3 * (size_t)v + q] = __fdiv_rn(s[q], val);
+}
+
+// subdivideMesh, step 1: head of each run of equal edges in the sorted encounters; the first
+// encounter of each edge marked in encounter order
+__global__ __launch_bounds__(BS) void k_edge_heads(const uint64_t* __restrict__ skey, const uint32_t* __restrict__ senc,
+                                                   uint32_t ne, uint32_t* __restrict__ headIdx,
+                                                   uint8_t* __restrict__ isFirst) {
+  const uint32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= ne) return;
+  const bool head = i == 0 || skey[i] != skey[i - 1];
+  headIdx[i] = head ? i : 0;
+  isFirst[senc[i]] = head ? 1 : 0;
+}
+
+// step 2: each encounter's new vertex (nv + rank of its edge's first encounter), and the midpoint
+// (P[u] + P[v]) / 2 of each edge at its head
+__global__ __launch_bounds__(BS) void k_edge_verts(const uint64_t* __restrict__ skey, const uint32_t* __restrict__ senc,
+                                                   const uint32_t* __restrict__ headScan, const uint32_t* __restrict__ rank,
+                                                   uint32_t ne, uint32_t nv, const float* __restrict__ P,
+                                                   uint32_t* __restrict__ newV, float* __restrict__ outP) {
+  const uint32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= ne) return;
+  const uint32_t h = headScan[i];
+  const uint32_t id = nv + rank[senc[h]];
+  newV[senc[i]] = id;
+  if (h == i) {
+    const uint32_t u = (uint32_t)(skey[i] >> 32), v = (uint32_t)skey[i];
+    for (int q = 0; q < 3; ++q)
+      outP[3 * (size_t)id + q] = __fdiv_rn(__fadd_rn(P[3 * (size_t)u + q], P[3 * (size_t)v + q]), 2.0f);
+  }
+}
+
+// step 3: triangle t becomes (a, ab, ca); (ab, b, bc), (ab, bc, ca), (ca, bc, c) are appended
+__global__ __launch_bounds__(BS) void k_subdiv_tris(const uint32_t* __restrict__ tri, uint32_t nt,
+                                                    const uint32_t* __restrict__ newV, uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * BS + threadIdx.x;
+  if (t >= nt) return;
+  const uint32_t a = tri[3 * (size_t)t], b = tri[3 * (size_t)t + 1], c = tri[3 * (size_t)t + 2];
+  const uint32_t ab = newV[3 * t], bc = newV[3 * t + 1], ca = newV[3 * t + 2];
+  uint32_t* o = out + 3 * (size_t)t;
+  o[0] = a, o[1] = ab, o[2] = ca;
+  uint32_t* q = out + 3 * ((size_t)nt + 3 * (size_t)t);
+  q[0] = ab, q[1] = b, q[2] = bc;
+  q[3] = ab, q[4] = bc, q[5] = ca;
+  q[6] = ca, q[7] = bc, q[8] = c;
+}
+
 template <class T>
 struct DVec {
   T* p = nullptr;
@@ -888,6 +983,147 @@ int mlod_dc_mesh(const mlod_dc* dc, float* verts, float* unit_verts, uint32_t* v
   if (tris && dc->nTris)
     MLOD_HIP(hipMemcpyAsync(tris, dc->u32.p, 3 * (size_t)dc->nTris * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   MLOD_HIP(hipStreamSynchronize(s));
+  return MLOD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+struct MeshBufs {
+  float* P = nullptr;
+  float* out = nullptr;
+  uint32_t* tri = nullptr;
+  uint32_t* otri = nullptr;
+  uint64_t* key = nullptr;
+  uint64_t* skey = nullptr;
+  uint32_t* val = nullptr;
+  uint32_t* sval = nullptr;
+  uint32_t* a = nullptr;
+  uint32_t* b = nullptr;
+  uint32_t* c = nullptr;
+  uint8_t* f = nullptr;
+  void* tmp = nullptr;
+  ~MeshBufs() {
+    void* p[] = {P, out, tri, otri, key, skey, val, sval, a, b, c, f, tmp};
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+  }
+};
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+int check_tris(const uint32_t* tris, uint32_t nt, uint32_t nv) {
+  for (size_t i = 0; i < 3 * (size_t)nt; ++i)
+    if (tris[i] >= nv) return fail(MLOD_ERR_INVALID, "mesh: a triangle index is not a vertex");
+  return MLOD_OK;
+}
+
+// sorted encounter keys (stable) for the mesh pass; returns MLOD_OK or an error
+int mesh_sort(MeshBufs& B, uint32_t nt, int edges, hipStream_t s) {
+  const size_t ne = 3 * (size_t)nt;
+  k_mesh_keys<<<blocks(ne), BS, 0, s>>>(B.tri, nt, edges, B.key, B.val);
+  MLOD_HIP(hipGetLastError());
+  size_t need = 0;
+  MLOD_HIP(rocprim::radix_sort_pairs(nullptr, need, B.key, B.skey, B.val, B.sval, ne, 0, 64, s));
+  MLOD_HIP(hipMalloc(&B.tmp, need + 256));
+  MLOD_HIP(rocprim::radix_sort_pairs(B.tmp, need, B.key, B.skey, B.val, B.sval, ne, 0, 64, s));
+  return MLOD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mlod_mesh_smooth(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out_verts, void* stream) {
+  if (!verts || !out_verts || (nt && !tris)) return fail(MLOD_ERR_INVALID, "NULL argument");
+  if (nt > 0x55555555u) return fail(MLOD_ERR_UNSUPPORTED, "mesh: too many triangles");
+  if (int r = check_tris(tris, nt, nv)) return r;
+  if (nv == 0) return MLOD_OK;
+  hipStream_t s = (hipStream_t)stream;
+  MeshBufs B;
+  const size_t ne = 3 * (size_t)nt;
+  MLOD_HIP(dalloc(&B.P, 3 * (size_t)nv));
+  MLOD_HIP(dalloc(&B.out, 3 * (size_t)nv));
+  MLOD_HIP(dalloc(&B.tri, ne));
+  MLOD_HIP(dalloc(&B.key, ne));
+  MLOD_HIP(dalloc(&B.skey, ne));
+  MLOD_HIP(dalloc(&B.val, ne));
+  MLOD_HIP(dalloc(&B.sval, ne));
+  MLOD_HIP(hipMemcpyAsync(B.P, verts, 3 * (size_t)nv * sizeof(float), hipMemcpyHostToDevice, s));
+  if (nt) {
+    MLOD_HIP(hipMemcpyAsync(B.tri, tris, ne * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (int r = mesh_sort(B, nt, 0, s)) return r;
+  }
+  k_smooth<<<blocks(nv), BS, 0, s>>>(B.P, nv, B.tri, B.skey, B.sval, (uint32_t)ne, B.out);
+  MLOD_HIP(hipGetLastError());
+  MLOD_HIP(hipMemcpyAsync(out_verts, B.out, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToHost, s));
+  MLOD_HIP(hipStreamSynchronize(s));
+  return MLOD_OK;
+}
+
+int mlod_mesh_subdivide(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out_verts,
+                        uint32_t out_cap, uint32_t* out_tris, uint32_t* out_nv, void* stream) {
+  if (!verts || !out_verts || !out_nv || (nt && (!tris || !out_tris))) return fail(MLOD_ERR_INVALID, "NULL argument");
+  if (nt > 0x15555555u) return fail(MLOD_ERR_UNSUPPORTED, "mesh: too many triangles");
+  if ((uint64_t)nv + 3ull * nt > 0xffffffffull) return fail(MLOD_ERR_UNSUPPORTED, "mesh: too many vertices");
+  if (out_cap < nv + 3 * nt) return fail(MLOD_ERR_INVALID, "mesh: out_verts holds fewer than nv + 3 nt vertices");
+  if (int r = check_tris(tris, nt, nv)) return r;
+  hipStream_t s = (hipStream_t)stream;
+  std::memcpy(out_verts, verts, 3 * (size_t)nv * sizeof(float));
+  *out_nv = nv;
+  if (nt == 0) return MLOD_OK;
+  MeshBufs B;
+  const size_t ne = 3 * (size_t)nt;
+  MLOD_HIP(dalloc(&B.P, 3 * (size_t)nv));
+  MLOD_HIP(dalloc(&B.out, 3 * ((size_t)nv + ne)));
+  MLOD_HIP(dalloc(&B.tri, ne));
+  MLOD_HIP(dalloc(&B.otri, 4 * ne));
+  MLOD_HIP(dalloc(&B.key, ne));
+  MLOD_HIP(dalloc(&B.skey, ne));
+  MLOD_HIP(dalloc(&B.val, ne));
+  MLOD_HIP(dalloc(&B.sval, ne));
+  MLOD_HIP(dalloc(&B.a, ne + 1));
+  MLOD_HIP(dalloc(&B.b, ne + 1));
+  MLOD_HIP(dalloc(&B.c, ne));
+  MLOD_HIP(dalloc(&B.f, ne + 1));
+  MLOD_HIP(hipMemcpyAsync(B.P, verts, 3 * (size_t)nv * sizeof(float), hipMemcpyHostToDevice, s));
+  MLOD_HIP(hipMemcpyAsync(B.tri, tris, ne * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  if (int r = mesh_sort(B, nt, 1, s)) return r;
+  // heads: a = head index per sorted position, f = first-encounter flags (encounter order)
+  MLOD_HIP(hipMemsetAsync(B.f, 0, ne + 1, s));
+  k_edge_heads<<<blocks(ne), BS, 0, s>>>(B.skey, B.sval, (uint32_t)ne, B.a, B.f);
+  MLOD_HIP(hipGetLastError());
+  // b = the head of each sorted position's run (max-scan of the head indices); c = rank of each
+  // first encounter (exclusive scan of the flags, ne + 1 entries: the last is the edge count)
+  size_t n1 = 0, n2 = 0;
+  MLOD_HIP(rocprim::inclusive_scan(nullptr, n1, B.a, B.b, ne, rocprim::maximum<uint32_t>(), s));
+  MLOD_HIP(rocprim::exclusive_scan(nullptr, n2, B.f, B.c, 0u, ne, rocprim::plus<uint32_t>(), s));
+  void* t2 = nullptr;
+  MLOD_HIP(hipMalloc(&t2, std::max(n1, n2) + 256));
+  size_t bytes = std::max(n1, n2);
+  hipError_t e = rocprim::inclusive_scan(t2, bytes, B.a, B.b, ne, rocprim::maximum<uint32_t>(), s);
+  bytes = std::max(n1, n2);
+  if (!e) e = rocprim::exclusive_scan(t2, bytes, B.f, B.c, 0u, ne, rocprim::plus<uint32_t>(), s);
+  uint32_t lastRank = 0;
+  uint8_t lastFlag = 0;
+  if (!e) e = hipMemcpyAsync(&lastRank, B.c + ne - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (!e) e = hipMemcpyAsync(&lastFlag, B.f + ne - 1, 1, hipMemcpyDeviceToHost, s);
+  if (!e) e = hipStreamSynchronize(s);
+  (void)hipFree(t2);
+  if (e) return hip_fail(e, "mesh subdivision", __FILE__, __LINE__);
+  const uint32_t nEdges = lastRank + lastFlag;
+  // B.val is free now: the new vertex of every encounter
+  MLOD_HIP(hipMemcpyAsync(B.out, B.P, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, s));
+  k_edge_verts<<<blocks(ne), BS, 0, s>>>(B.skey, B.sval, B.b, B.c, (uint32_t)ne, nv, B.P, B.val, B.out);
+  MLOD_HIP(hipGetLastError());
+  k_subdiv_tris<<<blocks(nt), BS, 0, s>>>(B.tri, nt, B.val, B.otri);
+  MLOD_HIP(hipGetLastError());
+  MLOD_HIP(hipMemcpyAsync(out_verts, B.out, 3 * ((size_t)nv + nEdges) * sizeof(float), hipMemcpyDeviceToHost, s));
+  MLOD_HIP(hipMemcpyAsync(out_tris, B.otri, 4 * ne * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  MLOD_HIP(hipStreamSynchronize(s));
+  *out_nv = nv + nEdges;
   return MLOD_OK;
 }
 

@@ -3,10 +3,10 @@ mesh output from the projection (scene::computeDualContouring, sources/scene/sce
 
 ``DualContouring(cells, max_depth, cut)`` uniformizes the cells (upsample.cpp:262-272);
 ``project_corners(tree)`` takes the corner field from the tree's projection (or
-``set_corner_values`` from the caller); ``extract()`` returns the mesh.  ``surface_mesh`` runs the
-reference's sequence on a model: adaptive octree, uniformize, corner field, extraction, and the
-vertices projected three steps onto the surface (scene.cpp:1417-1452, without its smoothing and
-subdivision passes)."""
+``set_corner_values`` from the caller); ``extract()`` returns the mesh.  ``smooth`` and
+``subdivide`` are the reference's mesh passes.  ``surface_mesh`` runs the reference's sequence on a
+model: adaptive octree, uniformize, corner field, extraction, vertices projected three steps onto
+the surface, then twice smoothing, subdivision and projection (scene.cpp:1417-1452)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -15,7 +15,7 @@ import numpy as np
 
 from ._lib import check, lib
 
-__all__ = ["DualContouring", "surface_mesh"]
+__all__ = ["DualContouring", "smooth", "subdivide", "surface_mesh"]
 
 
 def _p(a):
@@ -91,11 +91,33 @@ class DualContouring:
         return v, t, vn, u
 
 
-def surface_mesh(apss, depth, cut=False, reproject_steps=3, log=None):
+def smooth(V, F):
+    """mesh::smoothMesh (upsample.h:55-76) on the GPU, the reference's result bit for bit."""
+    V = np.ascontiguousarray(V, np.float32)
+    F = np.ascontiguousarray(F, np.uint32)
+    out = np.zeros_like(V)
+    check(lib().mlod_mesh_smooth(_p(V), len(V), _p(F), len(F), _p(out), None))
+    return out
+
+
+def subdivide(V, F):
+    """mesh::subdivideMesh (upsample.h:78-133) on the GPU: (vertices, triangles)."""
+    V = np.ascontiguousarray(V, np.float32)
+    F = np.ascontiguousarray(F, np.uint32)
+    cap = len(V) + 3 * len(F)
+    out = np.zeros((cap, 3), np.float32)
+    T = np.zeros((4 * len(F), 3), np.uint32)
+    n = C.c_uint32()
+    check(lib().mlod_mesh_subdivide(_p(V), len(V), _p(F), len(F), _p(out), cap, _p(T), C.byref(n), None))
+    return out[:n.value], T
+
+
+def surface_mesh(apss, depth, cut=False, reproject_steps=3, refine=2, log=None):
     """The reference's mesh of the surface for an APSS plugin `apss` (Fastapss or GlobalAPSS with
     its kernel and settings installed): OctreeGenerator.generate(depth) with its crossed cells,
     uniformize, corner field by projection, dual contouring, then the vertices projected
-    `reproject_steps` steps (scene.cpp:1417-1433).  Returns (vertices, normals, triangles, info)."""
+    `reproject_steps` steps, then `refine` rounds of smoothing, subdivision and projection
+    (scene.cpp:1417-1452).  Returns (vertices, normals, triangles, info)."""
     import time
     from .apss import OctreeGenerator
     t0 = time.perf_counter()
@@ -115,10 +137,17 @@ def surface_mesh(apss, depth, cut=False, reproject_steps=3, log=None):
     N = np.zeros_like(V)
     if reproject_steps and len(V):
         V, N = apss.project_points(V, reproject_steps)
+    for _ in range(refine):
+        if not len(F):
+            break
+        V = smooth(V, F)
+        V, F = subdivide(V, F)
+        V, N = apss.project_points(V, reproject_steps)
     t5 = time.perf_counter()
-    info = dict(cells=len(cells), nodes=inf["nodes"], leaves=inf["leaves"], corners=inf["corners"], vertices=len(V),
-                triangles=len(F), ms_generate=(t1 - t0) * 1e3, ms_uniformize=(t2 - t1) * 1e3,
-                ms_corners=(t3 - t2) * 1e3, ms_extract=(t4 - t3) * 1e3, ms_reproject=(t5 - t4) * 1e3)
+    info = dict(cells=len(cells), nodes=inf["nodes"], leaves=inf["leaves"], corners=inf["corners"], dc_vertices=len(VN),
+                dc_triangles=int(len(F) // 4 ** refine) if len(F) else 0, vertices=len(V), triangles=len(F),
+                ms_generate=(t1 - t0) * 1e3, ms_uniformize=(t2 - t1) * 1e3, ms_corners=(t3 - t2) * 1e3,
+                ms_extract=(t4 - t3) * 1e3, ms_refine=(t5 - t4) * 1e3)
     if log:
         log(f"surface mesh depth {depth}: {info['cells']} crossed cells -> {info['nodes']} nodes, {info['leaves']} "
             f"leaves, {info['corners']} corners -> {info['vertices']} vertices, {info['triangles']} triangles")

@@ -31,7 +31,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-__all__ = ["read_points", "write_points", "normalize", "Frame", "read_params", "Params", "write_timings"]
+__all__ = ["read_points", "write_points", "write_mesh", "read_mesh", "normalize", "Frame", "read_params", "Params",
+           "write_timings"]
 
 _PLY_TYPES = {
     "char": "i1", "int8": "i1", "uchar": "u1", "uint8": "u1", "short": "i2", "int16": "i2", "ushort": "u2",
@@ -195,6 +196,52 @@ def write_points(path: str, P: np.ndarray, N: np.ndarray | None = None, frame: F
         f.write(rec.astype("<f4").tobytes())
 
 
+def write_mesh(path: str, V: np.ndarray, N: np.ndarray, F: np.ndarray, frame: Frame | None = None,
+               comments: list | None = None):
+    """The surface mesh file of scene::computeDualContouring (scene.cpp:1489-1560): a binary
+    little-endian PLY with comment lines, vertices x y z nx ny nz (positions mapped back with the
+    frame) and faces as a uchar count 3 and three int indices."""
+    V = np.ascontiguousarray(V, np.float32)
+    if frame is not None:
+        V = (V * np.float32(frame.norme) + np.asarray(frame.offset, np.float32)).astype(np.float32)
+    N = np.ascontiguousarray(N, np.float32)
+    F = np.ascontiguousarray(F, np.uint32)
+    head = ["ply", "format binary_little_endian 1.0"] + [f"comment {c}" for c in (comments or [])] + [
+        f"element vertex {len(V)}", "property float x", "property float y", "property float z",
+        "property float nx", "property float ny", "property float nz", f"element face {len(F)}",
+        "property list uchar int vertex_indices", "end_header"]
+    face = np.zeros(len(F), dtype=np.dtype([("n", "u1"), ("v", "<i4", (3,))]))
+    face["n"] = 3
+    face["v"] = F.astype(np.int32)
+    with open(path, "wb") as f:
+        f.write(("\n".join(head) + "\n").encode("ascii"))
+        f.write(np.concatenate([V, N], 1).astype("<f4").tobytes())
+        f.write(face.tobytes())
+
+
+def read_mesh(path: str):
+    """Vertices, normals (or None) and triangles of a PLY mesh with a vertex and a face element."""
+    with open(path, "rb") as f:
+        fmt, elements = _parse_ply_header(f)
+        body = f.read()
+    if fmt != "binary_little_endian":
+        raise ValueError("read_mesh reads binary little-endian PLY")
+    P, N = _read_ply(path)
+    pos = 0
+    F = None
+    for el in elements:
+        if el["name"] == "vertex":
+            pos += sum(np.dtype(t).itemsize for _, t in el["props"]) * el["count"]
+        elif el["name"] == "face":
+            (_, (_, ct, it)), = el["props"]
+            dt = np.dtype([("n", ct), ("v", "<" + it, (3,))])
+            arr = np.frombuffer(body, dtype=dt, count=el["count"], offset=pos)
+            if np.any(arr["n"] != 3):
+                raise ValueError("read_mesh reads triangles")
+            F = arr["v"].astype(np.uint32)
+    return P, N, F
+
+
 # scene::loadParameters field order (scene.cpp:1183-1250): camera quaternion (4 floats) and
 # translation (3 floats) and distance; sigma; five fixed sigmas; beginningSigma; factor;
 # constant_shift; exponent; cutoff (the scaling-protection sphere); nbOfComputedSigmas;
@@ -272,10 +319,12 @@ def write_timings(path: str, model: str, n_input: int, n_projected: int, step_ms
 
 
 def project_model(model: str, params_path: str | None = None, out: str | None = None, precision="fp32c",
-                  semantics="ref_gpu", log=print):
+                  semantics="ref_gpu", log=print, mesh_depth: int | None = None):
     """The reference's automatic projection of a model's own points (mainwindow.cpp:180-220,
     scene.cpp:916-966) with its .params settings: read, normalise, build the octree, project the
-    points maxNbIterations steps, write <model>Points.ply (file frame) and <model>.timings."""
+    points maxNbIterations steps, write <model>Points.ply (file frame) and <model>.timings; with
+    mesh_depth (0: the .params maxDepth), also the dual-contouring surface mesh <model>Mesh.ply
+    (scene::computeDualContouring)."""
     from . import apss as A
     from . import hiprt
     P, N = read_points(model)
@@ -304,6 +353,13 @@ def project_model(model: str, params_path: str | None = None, out: str | None = 
         steps.append((time.perf_counter() - t0) * 1e3)
     base = os.path.splitext(out or model)[0]
     write_points(base + "Points.ply", op, on, frame)
+    if mesh_depth is not None:
+        from .dc import surface_mesh
+        V, N, F, info = surface_mesh(ap, mesh_depth if mesh_depth > 0 else prm.max_depth)
+        write_mesh(base + "Mesh.ply", V, N, F, frame, comments=["Created by mlodsurfaces_amd", "Octree:",
+                                                                 "Built using the adaptive octree method"])
+        log(f"{model}: surface mesh of {info['triangles']} triangles at depth {mesh_depth or prm.max_depth} -> "
+            f"{base}Mesh.ply")
     write_timings(base + ".timings", model, len(P), len(Pn), steps, gpu=True,
                   method="FastAPSS" if prm.method else "Global APSS", max_depth=prm.max_depth)
     log(f"{model}: {len(P)} points, {len(K.sigmas) or 1} sigmas, cutoff {prm.cutoff:g}, min depth {prm.min_depth}, "
@@ -318,8 +374,11 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="output base name (default: the model's)")
     ap.add_argument("--precision", default="fp32c", choices=["fp32", "fp32c", "fp64"])
     ap.add_argument("--semantics", default="ref_gpu", choices=["ref_gpu", "ref_cpu"])
+    ap.add_argument("--mesh", type=int, default=None, metavar="DEPTH",
+                    help="also write the surface mesh at this adaptive depth (0: the .params maxDepth)")
     a = ap.parse_args(argv)
-    project_model(a.model, a.params, a.out, a.precision, a.semantics, log=lambda m: print(m, file=sys.stderr))
+    project_model(a.model, a.params, a.out, a.precision, a.semantics, log=lambda m: print(m, file=sys.stderr),
+                  mesh_depth=a.mesh)
 
 
 if __name__ == "__main__":

@@ -110,6 +110,9 @@ def oracle_lib():
         L.orc_dc_set_corners.argtypes = [_vp, _i, _f, C.c_uint32]
         L.orc_dc_extract.argtypes = [_vp, _u, _u]
         L.orc_dc_mesh.argtypes = [_vp, _f, _u, _u]
+        L.orc_mesh_smooth.argtypes = [_f, C.c_uint32, _u, C.c_uint32]
+        L.orc_mesh_subdivide.restype = C.c_uint32
+        L.orc_mesh_subdivide.argtypes = [_f, C.c_uint32, _u, C.c_uint32]
         L.orc_cell_center.argtypes = [_i, _f, _f]
         L.orc_cell_ancestor.argtypes = [_i, C.c_int32, _i]
         _orc = L
@@ -405,3 +408,22 @@ class OracleDC:
         t = np.zeros((nt.value, 3), np.uint32)
         self.L.orc_dc_mesh(self.h, _ptr(v, C.c_float), _ptr(vn, C.c_uint32), _ptr(t, C.c_uint32))
         return v, vn, t
+
+
+def oracle_mesh_smooth(V, F):
+    L = oracle_lib()
+    P = np.ascontiguousarray(V, np.float32).copy()
+    T = np.ascontiguousarray(F, np.uint32)
+    L.orc_mesh_smooth(_ptr(P, C.c_float), len(P), _ptr(T, C.c_uint32), len(T))
+    return P
+
+
+def oracle_mesh_subdivide(V, F):
+    L = oracle_lib()
+    nv, nt = len(V), len(F)
+    P = np.zeros((nv + 3 * nt, 3), np.float32)
+    P[:nv] = V
+    T = np.zeros((4 * nt, 3), np.uint32)
+    T[:nt] = F
+    n = L.orc_mesh_subdivide(_ptr(P, C.c_float), nv, _ptr(T, C.c_uint32), nt)
+    return P[:n], T

@@ -360,4 +360,45 @@ void orc_dc_mesh(void* h, float* verts, uint32_t* vert_node, uint32_t* tris) {
   std::memcpy(tris, T->tris.data(), T->tris.size() * sizeof(uint32_t));
 }
 
+// mesh::smoothMesh (sources/scene/upsample.h:55-76), sequentially as the reference
+void orc_mesh_smooth(float* P, uint32_t nv, const uint32_t* T, uint32_t nt) {
+  std::vector<float> np(3 * (size_t)nv, 0.f), val(nv, 0.f);
+  for (uint32_t t = 0; t < nt; ++t) {
+    const uint32_t a = T[3 * t], b = T[3 * t + 1], c = T[3 * t + 2];
+    for (int q = 0; q < 3; ++q) {
+      const float ab = P[3 * b + q] + P[3 * c + q], ba = P[3 * a + q] + P[3 * c + q], ca = P[3 * b + q] + P[3 * a + q];
+      np[3 * a + q] += ab;
+      np[3 * b + q] += ba;
+      np[3 * c + q] += ca;
+    }
+    val[a] += 2, val[b] += 2, val[c] += 2;
+  }
+  for (uint32_t v = 0; v < nv; ++v)
+    for (int q = 0; q < 3; ++q) P[3 * v + q] = np[3 * v + q] / val[v];
+}
+
+// mesh::subdivideMesh (upsample.h:78-133): P holds room for nv + 3 nt vertices, T for 4 nt
+// triangles; returns the new vertex count
+uint32_t orc_mesh_subdivide(float* P, uint32_t nv, uint32_t* T, uint32_t nt) {
+  std::unordered_map<uint64_t, uint32_t> onEdge;
+  uint32_t n = nv;
+  size_t tn = 3 * (size_t)nt;
+  auto mid = [&](uint32_t u, uint32_t v) {
+    const uint64_t k = ((uint64_t)std::min(u, v) << 32) | std::max(u, v);
+    auto it = onEdge.find(k);
+    if (it != onEdge.end()) return it->second;
+    for (int q = 0; q < 3; ++q) P[3 * (size_t)n + q] = (P[3 * u + q] + P[3 * v + q]) / 2.f;
+    onEdge[k] = n;
+    return n++;
+  };
+  for (uint32_t t = 0; t < nt; ++t) {
+    const uint32_t a = T[3 * t], b = T[3 * t + 1], c = T[3 * t + 2];
+    const uint32_t ab = mid(a, b), bc = mid(c, b), ca = mid(c, a);
+    T[3 * t] = a, T[3 * t + 1] = ab, T[3 * t + 2] = ca;
+    const uint32_t add[9] = {ab, b, bc, ab, bc, ca, ca, bc, c};
+    for (int q = 0; q < 9; ++q) T[tn++] = add[q];
+  }
+  return n;
+}
+
 }  // extern "C"

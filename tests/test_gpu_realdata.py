@@ -61,7 +61,7 @@ def test_project_model_files(tmp_path):
     model = str(tmp_path / "Face.ply")
     pointio.write_points(model, g["points"], g["normals"])
     (tmp_path / "Face.params").write_bytes(g["params"].tobytes())
-    op, on, frame, prm = pointio.project_model(model, log=lambda m: None)
+    op, on, frame, prm = pointio.project_model(model, log=lambda m: None, mesh_depth=0)
     P, N = pointio.read_points(str(tmp_path / "FacePoints.ply"))
     assert len(P) == len(g["points"]) and np.isfinite(P).all()
     assert np.array_equal(P, (op * np.float32(frame.norme) + frame.offset).astype(np.float32))
@@ -73,3 +73,16 @@ def test_project_model_files(tmp_path):
     lines = open(str(tmp_path / "Face.timings")).read().splitlines()
     assert lines[1] == f"Nb of input points: {len(P)}" and lines[3] == "GPU - Mode: FastAPSS"
     assert os.path.getsize(str(tmp_path / "FacePoints.ply")) > 24 * len(P)
+
+    # the surface mesh at the model's own depth covers the scan: every sample lies within a few
+    # sampling gaps of a mesh vertex.  Face is an open scan, and the octree's coarse nodes carry
+    # the fitted spheres past its border, so part of the mesh lies away from the samples (the
+    # generator's crossed cells do too); only the covering direction is a property here.
+    V, N, F = pointio.read_mesh(str(tmp_path / "FaceMesh.ply"))
+    assert len(F) > 1000 and np.isfinite(V).all() and F.max() < len(V) and len(N) == len(V)
+    from scipy.spatial import cKDTree
+    gap = np.median(cKDTree(g["points"]).query(g["points"], k=2)[0][:, 1])
+    cover, _ = cKDTree(V).query(g["points"])
+    print(f"mesh {len(V)} vertices {len(F)} triangles; sample-to-mesh distance / gap: median "
+          f"{np.median(cover) / gap:.2f}, 90% {np.quantile(cover, 0.9) / gap:.2f}")
+    assert np.median(cover) < 2 * gap and np.quantile(cover, 0.9) < 4 * gap

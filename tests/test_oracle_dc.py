@@ -194,3 +194,30 @@ def test_closed_meshes(oracle_built, kind):
         if pts:
             assert np.allclose(v[i], np.mean(pts, 0), atol=1e-6)
         assert np.all(v[i] >= lo - 1e-7) and np.all(v[i] <= hi + 1e-7)
+
+
+def test_mesh_passes(oracle_built):
+    """smoothMesh / subdivideMesh restated: smoothing = mean of the other two vertices of every
+    incident triangle (an independent float64 scatter), subdivision = a midpoint per edge, numbered
+    by first use, four triangles per triangle, the closed mesh staying closed."""
+    f = sphere_sdf((0.5, 0.45, 0.55), 0.3)
+    dc, nodes, leaf, D, cs, vals, v, vn, t = run(oracle_built, uniform_ids(4), f)
+    S = oracle_built.oracle_mesh_smooth(v, t)
+    acc = np.zeros((len(v), 3))
+    cnt = np.zeros(len(v))
+    for a, b, c in t.astype(int):
+        acc[a] += v[b].astype(np.float64) + v[c]
+        acc[b] += v[a].astype(np.float64) + v[c]
+        acc[c] += v[a].astype(np.float64) + v[b]
+        cnt[[a, b, c]] += 2
+    assert np.allclose(S, acc / cnt[:, None], atol=1e-6)
+    P, T = oracle_built.oracle_mesh_subdivide(S, t)
+    edges = {}
+    for a, b, c in t.astype(int):
+        for u, w in ((a, b), (c, b), (c, a)):
+            edges.setdefault((min(u, w), max(u, w)), len(v) + len(edges))
+    assert len(P) == len(v) + len(edges) and len(T) == 4 * len(t)
+    for (u, w), i in edges.items():
+        assert np.array_equal(P[i], ((S[u] + S[w]) / np.float32(2)).astype(np.float32))
+    dup, counts, ne = manifold(T)
+    assert dup == 1 and counts == {2} and len(P) - ne + len(T) == 2

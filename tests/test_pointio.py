@@ -131,3 +131,18 @@ def test_oracle_matches_reference_on_real_models(oracle_built, name):
         assert np.array_equal(op.astype(np.float32), g[f"p{it}"]) and np.array_equal(on.astype(np.float32), g[f"n{it}"])
         op64, _ = o.project(g["queries"], k, par, n_iter=it, precision=1)
         assert (np.linalg.norm(op64 - g[f"p{it}"], axis=1) / diag).max() < 1e-5
+
+
+def test_mesh_file_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    V = rng.standard_normal((50, 3)).astype(np.float32)
+    N = rng.standard_normal((50, 3)).astype(np.float32)
+    F = rng.integers(0, 50, (30, 3)).astype(np.uint32)
+    fr = pointio.Frame(np.array([0.5, 1.0, -2.0], np.float32), 4.0)
+    path = str(tmp_path / "m.ply")
+    pointio.write_mesh(path, V, N, F, fr, comments=["test"])
+    P, N2, F2 = pointio.read_mesh(path)
+    assert np.array_equal(P, (V * np.float32(4.0) + fr.offset).astype(np.float32))
+    assert np.array_equal(N2, N) and np.array_equal(F2, F)
+    head = open(path, "rb").read().split(b"end_header\n")[0].split(b"\n")
+    assert b"element face 30" in head and b"property list uchar int vertex_indices" in head
