@@ -130,6 +130,37 @@ def adaptive_measure(A, ds, w, depth, semantics, precision):
             "ms": dt * 1e3}
 
 
+def mesh_measure(ap, depth, label):
+    """scene::computeDualContouring's sequence (dc.surface_mesh: adaptive octree, uniformize, corner
+    field, extraction, 3-step vertex projection, two rounds of smoothing, subdivision and projection)
+    at `depth` with `ap`'s tree and settings, after one untimed run; wall times of its stages, each
+    ending at a host synchronisation."""
+    from mlodsurfaces_amd import dc, hiprt
+    dc.surface_mesh(ap, depth)
+    hiprt.device_synchronize()
+    t0 = time.perf_counter()
+    V, N, F, info = dc.surface_mesh(ap, depth)
+    ms = (time.perf_counter() - t0) * 1e3
+    out = {"depth": depth, "ms": ms, **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in info.items()}}
+    log(f"surface mesh {label} depth {depth}: {info['cells']} crossed cells -> {info['nodes']} nodes, "
+        f"{info['corners']} corners, {info['dc_triangles']} triangles -> {info['triangles']} refined, {ms:.1f} ms "
+        f"(generate {info['ms_generate']:.1f}, uniformize {info['ms_uniformize']:.1f}, corners "
+        f"{info['ms_corners']:.1f}, extract {info['ms_extract']:.1f}, refine {info['ms_refine']:.1f})")
+    return out
+
+
+def workload_mesh_measure(A, ds, w, depth, semantics, precision):
+    """The surface mesh of the workload's cloud in the reference's normalised frame."""
+    p = ds.pn_normalize(w["points"])
+    t = A.APSSOctree().build(p, w["normals"])
+    k = A.Kernel(**w["kernel"])
+    ap = A.Fastapss(t, k)
+    ap.setScalingProtectionSphere(w["cutoff"])
+    ap.set_semantics(semantics)
+    ap.set_precision(precision)
+    return mesh_measure(ap, depth, "workload")
+
+
 def real_models_measure(A):
     """The reference's own sample models (Face; a 24k-point subset of half-igea) with their own
     .params settings: each model's points projected maxNbIterations steps (FP32C, REF_GPU) and
@@ -176,11 +207,13 @@ def real_models_measure(A):
         ap.copyPointsToGPU(len(g["queries"]), g["queries"])
         rp, _ = ap.project(len(g["queries"]), np.zeros_like(g["queries"]), np.zeros_like(g["queries"]), 3, K)
         rerr = np.linalg.norm(rp.astype(np.float64) - g["p3"], axis=1) / diag
+        ap.set_semantics("ref_gpu")
+        mesh = mesh_measure(ap, prm.max_depth, name)
         ms = res["fp32c"][1]
         out[name] = {"points": len(Pn), "sigmas": len(K.sigmas), "cutoff": prm.cutoff, "min_depth": prm.min_depth,
                      "steps": steps, "ms": ms, "pts_per_s": len(Pn) * steps / (ms * 1e-3),
                      "fp64_ms": res["fp64"][1], "fp32c_vs_fp64_max": float(err.max()),
-                     "ref_cpu_vs_reference_projectCPU_max": float(rerr.max())}
+                     "ref_cpu_vs_reference_projectCPU_max": float(rerr.max()), "surface_mesh": mesh}
         log(f"real model {name}: {len(Pn)} points, {len(K.sigmas)} sigmas, {steps} steps in {ms:.2f} ms "
             f"({out[name]['pts_per_s'] / 1e6:.1f} M pts/s; FP64 {res['fp64'][1]:.2f} ms), FP32C vs FP64 max "
             f"{err.max():.2e}, REF_CPU FP32C vs the reference's projectCPU max {rerr.max():.2e} x diag")
@@ -271,6 +304,8 @@ def main():
     ap.add_argument("--serve-clouds", type=int, default=64)
     ap.add_argument("--adaptive-depth", type=int, default=10,
                     help="max depth of the adaptive octree timed with the torus1m workload")
+    ap.add_argument("--mesh-depth", type=int, default=8,
+                    help="depth of the surface mesh of the workload's cloud (0: none)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain elementwise copy of 512 MiB three times (to scale the HBM counters)")
     args = ap.parse_args()
@@ -403,6 +438,9 @@ def main():
         adaptive = adaptive_measure(A, ds, w, args.adaptive_depth, args.semantics, args.precision)
     del tree, dP, dN, dQ
     hiprt.device_synchronize()
+    mesh = None
+    if args.mesh_depth > 0 and args.workload == "superquadric8m" and args.scale == 1.0:
+        mesh = workload_mesh_measure(A, ds, w, args.mesh_depth, args.semantics, args.precision)
     real = None
     if args.workload == "superquadric8m" and args.scale == 1.0:
         real = real_models_measure(A)
@@ -448,6 +486,7 @@ def main():
         "serving": serving,
         "adaptive_octree": adaptive,
         "real_models": real,
+        "surface_mesh": mesh,
     }
     print(json.dumps(out), flush=True)
 

@@ -77,6 +77,36 @@ def test_uniformize_and_analytic_field_match_oracle(oracle_built):
     assert np.array_equal(V, (m00 * U + m03).astype(np.float32))
 
 
+@pytest.mark.parametrize("kind", ["uniform", "cut"])
+def test_uniform_and_cut_octrees_match_oracle(oracle_built, kind):
+    """A uniform grid (one depth-0 cell, no grading) and an octree cut one level above its finest
+    cells, with an analytic field: nodes, leaves and mesh equal the oracle's."""
+    if kind == "uniform":
+        g = np.arange(16)
+        X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+        ids = np.stack([np.full(X.size, 4), X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.int32)
+        args = (0, False)
+    else:
+        ids = near_sphere_cells()
+        args = (6, True)  # the cells are re-rooted two levels deeper (depth 8): keep depth <= 6
+    dc = DualContouring(ids, *args)
+    nodes, leaf = dc.nodes()
+    o = oracle_built.OracleDC(ids, *args)
+    on, ol = o.nodes()
+    keep = inside(on)
+    assert np.array_equal(nodes, on[keep]) and np.array_equal(leaf, ol[keep])
+    if kind == "cut":
+        assert nodes[:, 0].max() == 6 and dc.info()["max_depth"] == 6
+    xyz, pos = dc.corners()
+    f = (np.linalg.norm(pos.astype(np.float64) - (0.1 if kind == "uniform" else 0.6), axis=1)
+         - (0.7 if kind == "uniform" else 1.0)).astype(np.float32)
+    dc.set_corner_values(f)
+    gpu = dc.extract()
+    o.set_corners(xyz, f)
+    assert len(gpu[1]) > 500
+    compare_meshes(gpu, o.extract(), nodes, on)
+
+
 def test_projection_field_pipeline(oracle_built):
     p, n = ds.sphere(40000, seed=5, noise=0.0)
     p = (p * 0.6 + np.array([0.05, -0.1, 0.1])).astype(np.float32)

@@ -155,7 +155,7 @@ def test_uniform_grid_matches_grid_formulation(oracle_built):
     assert len(t) > 100
 
 
-@pytest.mark.parametrize("kind", ["uniform", "graded"])
+@pytest.mark.parametrize("kind", ["uniform", "graded", "cut"])
 def test_closed_meshes(oracle_built, kind):
     f = sphere_sdf((0.5, 0.45, 0.55), 0.3)
     if kind == "uniform":
@@ -167,7 +167,13 @@ def test_closed_meshes(oracle_built, kind):
         P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
         near = np.abs(f(P)) < 0.04
         ids = np.concatenate([np.full((near.sum(), 1), d), np.floor(P[near] * 2 ** d).astype(int)], 1)
-    dc, nodes, leaf, D, cs, vals, v, vn, t = run(oracle_built, ids, f)
+    dc, nodes, leaf, D, cs, vals, v, vn, t = run(oracle_built, ids, f, cut=kind == "cut", max_depth=5)
+    if kind == "cut":
+        # cut (upsample.cpp:95-100) before the sibling sets: the uncut octree's nodes down to depth 5,
+        # whose deepest nodes become leaves
+        full, _ = oracle_built.OracleDC(ids).nodes()
+        assert D == 5 and np.array_equal(nodes, full[full[:, 0] <= 5])
+        assert leaf[nodes[:, 0] == 5].all()
     dup, counts, ne = manifold(t)
     assert dup == 1 and counts == {2}
     assert len(v) - ne + len(t) == 2
