@@ -345,6 +345,24 @@ def main():
 
     K = A.Kernel(**w["kernel"])
     mode = w.get("mode", "mlod")
+    if "dc_grid" in w:
+        # the queries: the dual-contouring grid's finest cell centres, from the adaptive octree
+        # generated with the MLoD projection of the same tree (before the kNN / ball mode is set)
+        from mlodsurfaces_amd import dc
+        g = w["dc_grid"]
+        ap_ = A.Fastapss(tree, K)
+        ap_.setScalingProtectionSphere(w["cutoff"])
+        ap_.set_semantics(args.semantics)
+        ap_.set_precision(args.precision)
+        tg = time.perf_counter()
+        Q = dc.grid_centres(ap_, g["depth"], g["cap"], g["seed"])
+        del ap_
+        w["queries"] = Q
+        m = len(Q)
+        dQ = hiprt.DeviceBuffer(Q)
+        hiprt.device_synchronize()
+        w["description"] += f", {m} DC-grid cell centres (adaptive octree depth {g['depth']}, uniformized)"
+        log(f"DC-grid queries: {m} cell centres in {time.perf_counter() - tg:.1f} s")
     if mode == "knn":
         tree.knnMode(True, w["knn"])
     elif mode == "ball":

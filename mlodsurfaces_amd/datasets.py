@@ -188,19 +188,23 @@ def make_workload(name: str, scale: float = 1.0):
         return dict(points=p, normals=nr, queries=q, kernel=dict(type=2, sigmas=sig), cutoff=2.0,
                     description=f"superquadric N=M={n}, multi-Gaussian S=6, cutoff 2, offsets +-0.02 diag")
     if name in ("c5_knn", "c5_ball"):
-        # global-APSS variants at scale: 20M points on the displaced superquadric with 10 holes;
-        # queries = centres of the depth-11 cells of [-1, 1]^3 enclosing the points
-        # (generateFromPoints, upsample.cpp:128-144), capped at 16M
+        # global-APSS variants at scale: 20M points on the displaced superquadric with 10 holes.
+        # The queries are the centres of the dual-contouring grid's finest cells (the adaptive
+        # octree generated to depth 11 with the MLoD projection, uniformized; dc.grid_centres),
+        # capped at 16M: they need the GPU, so the caller computes them from `dc_grid` once the
+        # octree is built.  `queries` holds, for callers without the GPU, the depth-11 cells of
+        # [-1, 1]^3 enclosing the points (generateFromPoints, upsample.cpp:128-144).
         n = max(64, int(20_000_000 * scale))
         p, nr = superquadric(n, seed=5, holes=10)
-        q = cell_centres_of_points(p, 11, cap=max(64, int(16_000_000 * scale)), seed=55)
+        cap = max(64, int(16_000_000 * scale))
+        q = cell_centres_of_points(p, 11, cap=cap, seed=55)
         R = bbox_radius(p)
         sig = [0.01 * 2.0 ** i * R for i in range(4)]
         mode = "knn" if name == "c5_knn" else "ball"
         return dict(points=p, normals=nr, queries=q, kernel=dict(type=2, sigmas=sig), cutoff=2.0, mode=mode, knn=32,
-                    ball_radius=-1.0,
-                    description=f"superquadric N={n} (10 holes), {len(q)} depth-11 cell centres, "
-                                f"{'kNN k=32' if mode == 'knn' else 'ball r=3 sigma0'}, multi-Gaussian S=4")
+                    ball_radius=-1.0, dc_grid=dict(depth=11, cap=cap, seed=55),
+                    description=f"superquadric N={n} (10 holes), {'kNN k=32' if mode == 'knn' else 'ball r=3 sigma0'}, "
+                                f"multi-Gaussian S=4")
     raise KeyError(name)
 
 

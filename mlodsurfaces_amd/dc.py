@@ -15,7 +15,7 @@ import numpy as np
 
 from ._lib import check, lib
 
-__all__ = ["DualContouring", "smooth", "subdivide", "surface_mesh"]
+__all__ = ["DualContouring", "grid_centres", "smooth", "subdivide", "surface_mesh"]
 
 
 def _p(a):
@@ -79,6 +79,19 @@ class DualContouring:
         check(lib().mlod_dc_corner_values(self.h, _p(f), self._stream))
         return f
 
+    def leaf_centres(self, depth=None):
+        """Centres of the leaves at `depth` of the uniformized octree (default: its finest depth,
+        which holds the generator's deepest cells), in the projection frame, in node order:
+        new2bbox applied to (2 p + 1) / 2^(d + 1) in float32 (scene.cpp:1300-1315)."""
+        inf = self.info()
+        ids, leaf = self.nodes()
+        d = inf["max_depth"] if depth is None else int(depth)
+        L = ids[leaf & (ids[:, 0] == d)][:, 1:]
+        m00 = np.float32(2.0 * inf["diag"])
+        m03 = (np.float32(2.0) * inf["origin"] - np.float32(1.0)).astype(np.float32)
+        u = (2 * L + 1).astype(np.float32) * np.float32(2.0 ** -(d + 1))
+        return (m00 * u + m03).astype(np.float32)
+
     def extract(self):
         """(vertices in the projection frame, triangles, node of each vertex, unit-frame vertices)"""
         nv, nt = C.c_uint32(), C.c_uint32()
@@ -89,6 +102,26 @@ class DualContouring:
         t = np.zeros((nt.value, 3), np.uint32)
         check(lib().mlod_dc_mesh(self.h, _p(v), _p(u), _p(vn), _p(t), self._stream))
         return v, t, vn, u
+
+
+def grid_centres(apss, depth, cap=None, seed=0):
+    """The query points of the dual-contouring grid: the adaptive octree of `apss` generated with
+    max depth `depth` (its crossed cells), uniformized, and the centres of the leaves at the
+    uniformized octree's finest depth (the generator's deepest cells, whatever the re-rooting),
+    in node order; with `cap`, a seeded random subset of at most `cap` of them, kept in node
+    order."""
+    from .apss import OctreeGenerator
+    gen = OctreeGenerator(apss, getattr(apss, "kernel", None))
+    total = []
+    gen.generate(depth, total)
+    gen.close()
+    dc = DualContouring(total[0], depth)
+    q = dc.leaf_centres()
+    dc.close()
+    if cap is not None and len(q) > cap:
+        keep = np.sort(np.random.default_rng(seed).choice(len(q), int(cap), replace=False))
+        q = q[keep]
+    return q
 
 
 def smooth(V, F):

@@ -10,7 +10,7 @@ import pytest
 
 from mlodsurfaces_amd import apss as A
 from mlodsurfaces_amd import datasets as ds
-from mlodsurfaces_amd.dc import DualContouring, smooth, subdivide, surface_mesh
+from mlodsurfaces_amd.dc import DualContouring, grid_centres, smooth, subdivide, surface_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -160,3 +160,33 @@ def test_projection_field_pipeline(oracle_built):
     r = np.linalg.norm(V4.astype(np.float64) - [0.05, -0.1, 0.1], axis=1)
     assert np.abs(r - 0.6).max() < 1e-3
     assert np.allclose(np.linalg.norm(N4, axis=1), 1.0, atol=1e-4)
+
+
+@pytest.mark.parametrize("shift", [0.0, 0.55])
+def test_grid_centres(shift):
+    """The DC grid's query points: the uniformized octree's finest leaves are the generator's
+    deepest crossed cells and their siblings (grading and regularize_up only add coarser cells);
+    their centres, mapped back through the re-rooting frame, are aabb_octree::center of those
+    cells exactly, also when the cloud leaves [-1, 1]^3 and the octree is re-rooted."""
+    p, n = ds.sphere(30000, seed=7, noise=0.0)
+    p = (p * 0.6 + np.array([shift, -0.1, 0.1])).astype(np.float32)
+    tree = A.APSSOctree().build(p, n)
+    k = A.Kernel(type=2, sigmas=[float(np.float32(0.01 * 2 ** i) * np.float32(tree.radius())) for i in range(4)])
+    ap = A.Fastapss(tree, k)
+    ap.setScalingProtectionSphere(2.0)
+    gen = A.OctreeGenerator(ap, k)
+    total = []
+    gen.generate(6, total)
+    cells = total[0]
+    d = cells[:, 0].max()
+    deep = cells[cells[:, 0] == d][:, 1:]
+    fam = np.unique((((deep >> 1) << 1)[:, None, :] +
+                     np.array([[c & 1, (c >> 1) & 1, (c >> 2) & 1] for c in range(8)])[None]).reshape(-1, 3), axis=0)
+    expect = ((2 * fam + 1).astype(np.float32) * np.float32(2.0 ** -d) - np.float32(1.0)).astype(np.float32)
+    q = grid_centres(ap, 6)
+    if shift:
+        assert (np.abs(p) > 1).any() and DualContouring(cells, 6).info()["max_depth"] > d
+    assert len(q) == len(expect)
+    assert {tuple(r) for r in q.tolist()} == {tuple(r) for r in expect.tolist()}
+    capped = grid_centres(ap, 6, cap=100, seed=3)
+    assert len(capped) == 100 and {tuple(r) for r in capped.tolist()} <= {tuple(r) for r in q.tolist()}
