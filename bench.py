@@ -304,6 +304,10 @@ def main():
     ap.add_argument("--serve-clouds", type=int, default=64)
     ap.add_argument("--adaptive-depth", type=int, default=10,
                     help="max depth of the adaptive octree timed with the torus1m workload")
+    ap.add_argument("--c5-queries", default="dc-grid", choices=["dc-grid", "point-cells"],
+                    help="C5 queries: the dual-contouring grid's finest cell centres, or the depth-11 cells "
+                         "that contain a point")
+    ap.add_argument("--no-real-models", action="store_true", help="skip the reference's sample models")
     ap.add_argument("--mesh-depth", type=int, default=8,
                     help="depth of the surface mesh of the workload's cloud (0: none)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -328,6 +332,8 @@ def main():
     n, m = len(P), len(Q)
     lo, hi = P.min(0).astype(np.float64), P.max(0).astype(np.float64)
     diag = float(np.linalg.norm(hi - lo))
+    if "dc_grid" in w and args.c5_queries == "point-cells":
+        w["description"] += f", {m} depth-11 cells containing a point"
     log(f"workload {w['description']} generated in {time.time() - t0:.1f}s")
 
     stream = hiprt.Stream()
@@ -345,7 +351,7 @@ def main():
 
     K = A.Kernel(**w["kernel"])
     mode = w.get("mode", "mlod")
-    if "dc_grid" in w:
+    if "dc_grid" in w and args.c5_queries == "dc-grid":
         # the queries: the dual-contouring grid's finest cell centres, from the adaptive octree
         # generated with the MLoD projection of the same tree (before the kNN / ball mode is set)
         from mlodsurfaces_amd import dc
@@ -460,7 +466,7 @@ def main():
     if args.mesh_depth > 0 and args.workload == "superquadric8m" and args.scale == 1.0:
         mesh = workload_mesh_measure(A, ds, w, args.mesh_depth, args.semantics, args.precision)
     real = None
-    if args.workload == "superquadric8m" and args.scale == 1.0:
+    if args.workload == "superquadric8m" and args.scale == 1.0 and not args.no_real_models:
         real = real_models_measure(A)
     serving = None
     if not args.no_serve and args.workload == "superquadric8m" and args.scale == 1.0:
