@@ -250,6 +250,7 @@ class APSS {
 
  protected:
   friend class OctreeGenerator;
+  friend class DualContouring;
   APSS(APSSOctree* octree, int mode) : m_apssOctree(octree), m_mode(mode) {
     m_scalingProtectionSphere = octree->getScalingProtectionSphere();
     m_minDepth = octree->getMinDepth();

@@ -6,7 +6,8 @@
 //   records x y z nx ny nz) and PLY files (binary little-endian or ASCII; vertex properties
 //   x y z [nx ny nz] of any scalar type; later elements ignored).  save() writes the reference's
 //   PLY header and records with positions mapped back as p * norme + offset, as Pn::save and
-//   scene::saveProjectedPoints (sources/scene/scene.cpp:1622-1661) do.
+//   scene::saveProjectedPoints (sources/scene/scene.cpp:1622-1661) do; saveMesh() writes the
+//   surface mesh file of scene::computeDualContouring (scene.cpp:1489-1560).
 // * mlod::ModelParams — the 134-byte .params file of a model (scene::loadParameters,
 //   scene.cpp:1183-1250), and kernel(radius): the mlod::Kernel it describes, sigmas generated
 //   and scaled as Kernel::updateSigmas (Kernel.cpp:110-128).  Include mlod_apss.hpp first to get
@@ -105,6 +106,32 @@ class Pn {
       f.write(reinterpret_cast<const char*>(&p), sizeof p);
       if (!nrm.empty()) f.write(reinterpret_cast<const char*>(&nrm[i]), sizeof(Vec3f));
       else if (!ply) f.write("\0\0\0\0\0\0\0\0\0\0\0\0", sizeof(Vec3f));
+    }
+  }
+
+  // The surface mesh file of scene::computeDualContouring (scene.cpp:1489-1560): binary
+  // little-endian PLY, "comment" lines, vertices x y z nx ny nz with positions mapped back to the
+  // file frame, faces as a uchar 3 and three int indices.
+  void saveMesh(const std::string& path, const std::vector<Vec3f>& verts, const std::vector<Vec3f>& nrm,
+                const std::vector<uint32_t>& indices, const std::vector<std::string>& comments) const {
+    if (nrm.size() != verts.size()) throw std::runtime_error("saveMesh: one normal per vertex expected");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot write " + path);
+    f << "ply\nformat binary_little_endian 1.0\n";
+    for (const std::string& c : comments) f << "comment " << c << "\n";
+    f << "element vertex " << verts.size()
+      << "\nproperty float x\nproperty float y\nproperty float z\nproperty float nx\nproperty float ny\n"
+         "property float nz\nelement face "
+      << indices.size() / 3 << "\nproperty list uchar int vertex_indices\nend_header\n";
+    for (size_t i = 0; i < verts.size(); ++i) {
+      const Vec3f p{verts[i].x * norme + offset.x, verts[i].y * norme + offset.y, verts[i].z * norme + offset.z};
+      f.write(reinterpret_cast<const char*>(&p), sizeof p);
+      f.write(reinterpret_cast<const char*>(&nrm[i]), sizeof(Vec3f));
+    }
+    for (size_t t = 0; t + 2 < indices.size(); t += 3) {
+      const char three = 3;
+      f.write(&three, 1);
+      f.write(reinterpret_cast<const char*>(&indices[t]), 3 * sizeof(uint32_t));
     }
   }
 

@@ -31,8 +31,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-__all__ = ["read_points", "write_points", "write_mesh", "read_mesh", "normalize", "Frame", "read_params", "Params",
-           "write_timings"]
+__all__ = ["read_points", "write_points", "write_mesh", "mesh_comments", "read_mesh", "normalize", "Frame",
+           "read_params", "Params", "write_timings"]
 
 _PLY_TYPES = {
     "char": "i1", "int8": "i1", "uchar": "u1", "uint8": "u1", "short": "i2", "int16": "i2", "ushort": "u2",
@@ -219,6 +219,25 @@ def write_mesh(path: str, V: np.ndarray, N: np.ndarray, F: np.ndarray, frame: Fr
         f.write(face.tobytes())
 
 
+def mesh_comments(kernel, radius: float) -> list:
+    """The comment lines the reference writes into its mesh file (scene.cpp:1498-1530): the
+    kernel's parameters (as C++ streams print floats: six significant digits) and the octree
+    method, for a kernel whose sigmas were scaled by a cloud of bounding-box radius `radius`."""
+    f32 = np.float32
+    g = lambda v: format(float(f32(v)), "g")  # noqa: E731
+    c = ["Created by mlodsurfaces_amd", "Kernel parameters:"]
+    if kernel.type == 2:
+        c += ["Multiple Gaussian", f"First sigma  {g(kernel.beginningSigma)}", f"Factor {g(kernel.factor)}",
+              f"Number of sigmas {len(kernel.sigmas)}", f"Constant_shift (unused) {g(kernel.constant_shift)}"]
+    elif kernel.type == 1:
+        c += ["Singular", f"Exponent {g(kernel.exponent)}", f"Constant_shift {g(kernel.constant_shift)}"]
+    elif kernel.type == 0:
+        c += ["Gaussian", f"Sigma {g(f32(kernel.sigma) / f32(radius))}", f"Constant_shift {g(kernel.constant_shift)}"]
+    else:
+        c += ["Kernel unknown"]
+    return c + ["Octree:", "Built using the adaptive octree method"]
+
+
 def read_mesh(path: str):
     """Vertices, normals (or None) and triangles of a PLY mesh with a vertex and a face element."""
     with open(path, "rb") as f:
@@ -356,8 +375,7 @@ def project_model(model: str, params_path: str | None = None, out: str | None = 
     if mesh_depth is not None:
         from .dc import surface_mesh
         V, N, F, info = surface_mesh(ap, mesh_depth if mesh_depth > 0 else prm.max_depth)
-        write_mesh(base + "Mesh.ply", V, N, F, frame, comments=["Created by mlodsurfaces_amd", "Octree:",
-                                                                 "Built using the adaptive octree method"])
+        write_mesh(base + "Mesh.ply", V, N, F, frame, comments=mesh_comments(K, tree.radius()))
         log(f"{model}: surface mesh of {info['triangles']} triangles at depth {mesh_depth or prm.max_depth} -> "
             f"{base}Mesh.ply")
     write_timings(base + ".timings", model, len(P), len(Pn), steps, gpu=True,

@@ -1,9 +1,9 @@
 // Test driver for include/mlod_pn.hpp (no GPU): pn_tool <in.ply|pn> <out_prefix> [<model.params>]
 // writes <out_prefix>.norm (float32: offset[3], norme, then the normalised positions), saves the
 // normalised set back to <out_prefix>.ply (file frame) and prints the .params fields and the
-// kernel's sigmas for radius 0.8125.
-#include "mlod_apss.hpp"
-#include "mlod_pn.hpp"
+// kernel's sigmas for radius 0.8125; with the .params, also writes <out_prefix>Mesh.ply, a mesh
+// of the first four points (two triangles) with the mesh-file comments of that kernel.
+#include "mlod_dc.hpp"
 
 #include <cstdio>
 
@@ -30,6 +30,11 @@ int main(int argc, char** argv) {
     std::printf("sigma %.9g sigmas", k.sigma);
     for (float s : k.sigmas) std::printf(" %.9g", s);
     std::printf("\n");
+    if (pn.positions.size() >= 4) {
+      const std::vector<mlod::Vec3f> v(pn.positions.begin(), pn.positions.begin() + 4);
+      const std::vector<mlod::Vec3f> n(pn.normals.begin(), pn.normals.begin() + 4);
+      pn.saveMesh(out + "Mesh.ply", v, n, {0, 1, 2, 0, 2, 3}, mlod::meshComments(k, 0.8125f));
+    }
   }
   return 0;
 }

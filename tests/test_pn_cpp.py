@@ -57,6 +57,10 @@ def test_cpp_matches_python(tool, tmp_path, name, ext):
     vals = out[2].split()
     assert np.float32(vals[1]) == np.float32(k.sigma)
     assert np.array_equal(np.array(vals[3:], np.float32), np.array(k.sigmas, np.float32))
+    # Pn::saveMesh with meshComments = pointio.write_mesh with mesh_comments, byte for byte
+    pointio.write_mesh(str(tmp_path / "pyMesh.ply"), Pn[:4], g["normals"][:4], np.array([[0, 1, 2], [0, 2, 3]]), fr,
+                       comments=pointio.mesh_comments(k, 0.8125))
+    assert open(str(tmp_path / "cMesh.ply"), "rb").read() == open(str(tmp_path / "pyMesh.ply"), "rb").read()
 
 
 def test_cpp_ascii_ply(tool, tmp_path):
@@ -70,6 +74,41 @@ def test_cpp_ascii_ply(tool, tmp_path):
     Pn, fr = pointio.normalize(P)
     raw = np.fromfile(str(tmp_path / "c.norm"), np.float32)
     assert np.array_equal(raw[4:].reshape(-1, 3), Pn) and raw[3] == np.float32(fr.norme)
+
+
+def test_mesh_comments_kernel_types():
+    """The reference's kernel lines (scene.cpp:1498-1530) for the three kernel types."""
+    from mlodsurfaces_amd.apss import Kernel
+    c = pointio.mesh_comments(Kernel(type=2, sigmas=[0.1, 0.2], beginningSigma=0.01, factor=2.0), 1.0)
+    assert c[2:7] == ["Multiple Gaussian", "First sigma  0.01", "Factor 2", "Number of sigmas 2",
+                      "Constant_shift (unused) 0"]
+    assert pointio.mesh_comments(Kernel(type=1, exponent=4.0, constant_shift=1e-3), 1.0)[2:5] == \
+        ["Singular", "Exponent 4", "Constant_shift 0.001"]
+    assert pointio.mesh_comments(Kernel(type=0, sigma=0.05), 0.5)[2:4] == ["Gaussian", "Sigma 0.1"]
+    assert pointio.mesh_comments(Kernel(type=0), 1.0)[-2:] == ["Octree:", "Built using the adaptive octree method"]
+
+
+@pytest.mark.gpu
+def test_cpp_surface_mesh_matches_python(tmp_path):
+    """tests/cpp/dc_mesh.cpp (mlod_dc.hpp's computeDualContouring and Pn::saveMesh) writes the
+    same mesh file, byte for byte, as pointio.project_model(..., mesh_depth=0) for the Face model."""
+    from mlodsurfaces_amd import _lib
+    exe = str(tmp_path / "dc_mesh")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "dc_mesh.cpp"), "-o", exe, f"-L{libdir}", "-lmlod",
+                    f"-Wl,-rpath,{libdir}"], check=True, capture_output=True, text=True)
+    g = load_golden("real_Face.npz")
+    model = str(tmp_path / "Face.ply")
+    pointio.write_points(model, g["points"], g["normals"])
+    (tmp_path / "Face.params").write_bytes(g["params"].tobytes())
+    r = subprocess.run([exe, model, str(tmp_path / "Face.params"), str(tmp_path / "cpp.ply")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    pointio.project_model(model, log=lambda m: None, mesh_depth=0)
+    a = open(str(tmp_path / "cpp.ply"), "rb").read()
+    b = open(str(tmp_path / "FaceMesh.ply"), "rb").read()
+    assert len(a) > 100000 and a == b
 
 
 @pytest.mark.gpu
