@@ -31,7 +31,13 @@ namespace mlod {
 namespace {
 
 constexpr int PBS = 128;         // lanes per workgroup
-constexpr int kLevels = kMaxDepth;  // internal nodes have depth 0..11
+// Levels of the LDS stack of waiting nodes.  A node waits only while it has more than one near
+// child still to expand, and near children are internal nodes; internal nodes have depth
+// 0..kMaxDepth - 1 and their children depth <= kMaxDepth, so a waiting node has depth at most
+// kMaxDepth - 2 (its near children are internal, hence of depth <= kMaxDepth - 1).  One level
+// fewer than the depth count keeps the per-workgroup LDS at 11 x 9 x 128 B, which lets one more
+// workgroup per CU reside (six waves per SIMD, as many as the kernel's registers allow).
+constexpr int kLevels = kMaxDepth - 1;
 
 #define LAUNCH_CHECK()                                                           \
   do {                                                                           \
