@@ -255,6 +255,17 @@ int mlod_dc_extract(mlod_dc* dc, uint32_t* n_verts, uint32_t* n_tris, void* stre
  * triangles (n x 3 vertex indices); any may be NULL */
 int mlod_dc_mesh(const mlod_dc* dc, float* verts, float* unit_verts, uint32_t* vert_node, uint32_t* tris, void* stream);
 
+/* The surface mesh of scene::computeDualContouring after extraction (scene.cpp:1417-1452), kept on
+ * the device: the extracted vertices projected `steps` steps on tree t (its kernel, settings and
+ * mode; the default step bound), then `rounds` times smoothing, subdivision and projection again
+ * (the reference: steps 3, rounds 2).  The same arithmetic as mlod_mesh_smooth,
+ * mlod_mesh_subdivide and mlod_project_points on the host arrays, so the same bits.
+ * mlod_dc_refined_mesh copies vertices, normals (n x 3 floats, projection frame) and triangles
+ * (n x 3); any may be NULL. */
+int mlod_dc_refine(mlod_dc* dc, mlod_tree* t, uint32_t steps, uint32_t rounds, uint32_t* n_verts, uint32_t* n_tris,
+                   void* stream);
+int mlod_dc_refined_mesh(const mlod_dc* dc, float* verts, float* normals, uint32_t* tris, void* stream);
+
 /* Mesh passes of the surface output (host arrays in and out), the reference's results bit for bit:
  * mesh::smoothMesh (upsample.h:55-76): every vertex moved to the mean of the other vertices of
  * its triangles (NaN for a vertex no triangle uses); mesh::subdivideMesh (upsample.h:78-133): a

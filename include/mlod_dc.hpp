@@ -62,6 +62,20 @@ class DualContouring {
   // or the caller's values, one per corner in the order of the library's corner list
   void setCornerValues(const std::vector<float>& f) { check(mlod_dc_set_corner_values(m_dc, f.data(), m_stream)); }
 
+  // the extracted mesh refined on the device (scene.cpp:1417-1452): vertices projected `steps`
+  // steps with apss's tree and settings, then `rounds` times smoothing, subdivision, projection
+  void refine(const APSS& apss, unsigned steps, unsigned rounds, std::vector<Vec3f>& vertices,
+              std::vector<Vec3f>& normals, std::vector<uint32_t>& indices) {
+    apss.sync();
+    uint32_t nv = 0, nt = 0;
+    check(mlod_dc_refine(m_dc, apss.m_apssOctree->handle(), steps, rounds, &nv, &nt, m_stream));
+    vertices.assign(nv, Vec3f{});
+    normals.assign(nv, Vec3f{});
+    indices.assign(3 * (size_t)nt, 0);
+    check(mlod_dc_refined_mesh(m_dc, nv ? &vertices[0].x : nullptr, nv ? &normals[0].x : nullptr,
+                               nt ? indices.data() : nullptr, m_stream));
+  }
+
   // dual_contouring::extract: vertices in the projection frame, triangles (3 indices each)
   void extract(std::vector<Vec3f>& vertices, std::vector<uint32_t>& indices) {
     uint32_t nv = 0, nt = 0;
@@ -116,19 +130,7 @@ inline void computeDualContouring(const APSS& apss, const Kernel& kernel, unsign
   DualContouring dc(total, maxDepth, cut);
   dc.projectCorners(apss);
   dc.extract(vertices, indices);
-  normals.assign(vertices.size(), Vec3f{});
-  auto reproject = [&]() {
-    if (reprojectSteps && !vertices.empty())
-      apss.projectPoints((unsigned)vertices.size(), vertices.data(), vertices.data(), normals.data(), reprojectSteps,
-                         kernel);
-  };
-  reproject();
-  for (unsigned i = 0; i < refine && !indices.empty(); ++i) {
-    mesh::smoothMesh(vertices, indices);
-    mesh::subdivideMesh(vertices, indices);
-    normals.assign(vertices.size(), Vec3f{});
-    reproject();
-  }
+  dc.refine(apss, reprojectSteps, refine, vertices, normals, indices);
 }
 
 // The comment lines of the reference's mesh file (scene.cpp:1498-1530) for a kernel whose sigmas

@@ -93,6 +93,8 @@ SIGNATURES = {
     "mlod_dc_corner_values": (C.c_int, [_vp, _vp, _vp]),
     "mlod_dc_extract": (C.c_int, [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _vp]),
     "mlod_dc_mesh": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "mlod_dc_refine": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _vp]),
+    "mlod_dc_refined_mesh": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "mlod_mesh_smooth": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint32, _vp, _vp]),
     "mlod_mesh_subdivide": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, _vp, C.POINTER(C.c_uint32), _vp]),
 }

@@ -511,7 +511,10 @@ struct mlod_dc {
   mlod::DVec<uint64_t> keys, work, ck;
   mlod::DVec<uint8_t> leaf, used, flags8;
   mlod::DVec<uint32_t> leaves, vid, tri, vnode, u32;
-  mlod::DVec<float> cf, cpos, verts, vframe;
+  mlod::DVec<float> cf, cpos, verts, vframe, rv, rn;
+  mlod::DVec<uint32_t> rt;
+  uint32_t nRV = 0, nRT = 0;
+  bool refined = false;
   void* tmp = nullptr;
   size_t tmpBytes = 0;
   uint32_t* dCount = nullptr;
@@ -626,7 +629,7 @@ void frame_matrix(const mlod_dc* dc, float4& m00, float4& m03) {
 }
 
 int build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t maxDepth, bool cut, hipStream_t s) {
-  dc->extracted = dc->haveValues = false;
+  dc->extracted = dc->haveValues = dc->refined = false;
   dc->nLeaves = dc->nVerts = dc->nTris = 0;
   MLOD_HIP(hipMemsetAsync(dc->dFlags, 0, sizeof(uint32_t), s));
   if (int r = dc->keys.reserve(2 * (size_t)n + 1, 0, s)) return r;
@@ -755,6 +758,7 @@ int build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t maxDepth, bool 
 }
 
 int extract(mlod_dc* dc, hipStream_t s) {
+  dc->refined = false;
   const uint32_t nk = (uint32_t)dc->keys.n, nl = dc->nLeaves, nc = (uint32_t)dc->ck.n;
   if (int r = dc->tri.reserve((size_t)nl * 12 * 6, 0, s)) return r;
   if (int r = dc->used.reserve(nk, 0, s)) return r;
@@ -791,7 +795,7 @@ int extract(mlod_dc* dc, hipStream_t s) {
   dc->nTris = 0;
   const size_t slots = (size_t)nl * 24;
   if (slots) {
-    if (slots > 0xffffffffu) return fail(MLOD_ERR_UNSUPPORTED, "dual contouring: too many leaf edges");
+    if (3 * slots > 0xffffffffu) return fail(MLOD_ERR_UNSUPPORTED, "dual contouring: too many leaf edges");
     if (int r = dc->u32.reserve(3 * slots, 0, s)) return r;
     // a triangle is emitted when its first index is set; select whole triangles with a flag per index
     if (int r = dc->flags8.reserve(3 * slots, 0, s)) return r;
@@ -851,6 +855,7 @@ int mlod_dc_destroy(mlod_dc* dc) {
   dc->leaf.release(), dc->used.release(), dc->flags8.release();
   dc->leaves.release(), dc->vid.release(), dc->tri.release(), dc->vnode.release(), dc->u32.release();
   dc->cf.release(), dc->cpos.release(), dc->verts.release(), dc->vframe.release();
+  dc->rv.release(), dc->rn.release(), dc->rt.release();
   void* p[] = {dc->tmp, dc->dCount, dc->dCount64, dc->dFlags, dc->dBounds};
   for (void* q : p)
     if (q) (void)hipFree(q);
@@ -989,11 +994,8 @@ int mlod_dc_mesh(const mlod_dc* dc, float* verts, float* unit_verts, uint32_t* v
 }  // extern "C"
 
 namespace {
+// temporaries of one mesh pass
 struct MeshBufs {
-  float* P = nullptr;
-  float* out = nullptr;
-  uint32_t* tri = nullptr;
-  uint32_t* otri = nullptr;
   uint64_t* key = nullptr;
   uint64_t* skey = nullptr;
   uint32_t* val = nullptr;
@@ -1003,8 +1005,10 @@ struct MeshBufs {
   uint32_t* c = nullptr;
   uint8_t* f = nullptr;
   void* tmp = nullptr;
+  void* tmp2 = nullptr;
   ~MeshBufs() {
-    void* p[] = {P, out, tri, otri, key, skey, val, sval, a, b, c, f, tmp};
+    (void)hipDeviceSynchronize();
+    void* p[] = {key, skey, val, sval, a, b, c, f, tmp, tmp2};
     for (void* q : p)
       if (q) (void)hipFree(q);
   }
@@ -1021,16 +1025,96 @@ int check_tris(const uint32_t* tris, uint32_t nt, uint32_t nv) {
   return MLOD_OK;
 }
 
-// sorted encounter keys (stable) for the mesh pass; returns MLOD_OK or an error
-int mesh_sort(MeshBufs& B, uint32_t nt, int edges, hipStream_t s) {
+// sorted encounter keys (stable) of the triangles dTri for the mesh pass
+int mesh_sort(MeshBufs& B, const uint32_t* dTri, uint32_t nt, int edges, hipStream_t s) {
   const size_t ne = 3 * (size_t)nt;
-  k_mesh_keys<<<blocks(ne), BS, 0, s>>>(B.tri, nt, edges, B.key, B.val);
+  MLOD_HIP(dalloc(&B.key, ne));
+  MLOD_HIP(dalloc(&B.skey, ne));
+  MLOD_HIP(dalloc(&B.val, ne));
+  MLOD_HIP(dalloc(&B.sval, ne));
+  if (!nt) return MLOD_OK;
+  k_mesh_keys<<<blocks(ne), BS, 0, s>>>(dTri, nt, edges, B.key, B.val);
   MLOD_HIP(hipGetLastError());
   size_t need = 0;
   MLOD_HIP(rocprim::radix_sort_pairs(nullptr, need, B.key, B.skey, B.val, B.sval, ne, 0, 64, s));
   MLOD_HIP(hipMalloc(&B.tmp, need + 256));
   MLOD_HIP(rocprim::radix_sort_pairs(B.tmp, need, B.key, B.skey, B.val, B.sval, ne, 0, 64, s));
   return MLOD_OK;
+}
+
+// mesh::smoothMesh on device arrays: dP (nv x 3), dTri (nt x 3) -> dOut (nv x 3)
+int mesh_smooth_dev(const float* dP, uint32_t nv, const uint32_t* dTri, uint32_t nt, float* dOut, hipStream_t s) {
+  if (nv == 0) return MLOD_OK;
+  MeshBufs B;
+  if (int r = mesh_sort(B, dTri, nt, 0, s)) return r;
+  k_smooth<<<blocks(nv), BS, 0, s>>>(dP, nv, dTri, B.skey, B.sval, (uint32_t)(3 * (size_t)nt), dOut);
+  MLOD_HIP(hipGetLastError());
+  MLOD_HIP(hipStreamSynchronize(s));
+  return MLOD_OK;
+}
+
+// mesh::subdivideMesh on device arrays: dP (nv x 3), dTri (nt x 3) -> dOut (room for nv + 3 nt
+// vertices; *outNv written), dOutTri (4 nt x 3)
+int mesh_subdivide_dev(const float* dP, uint32_t nv, const uint32_t* dTri, uint32_t nt, float* dOut, uint32_t* dOutTri,
+                       uint32_t* outNv, hipStream_t s) {
+  *outNv = nv;
+  if (nv) MLOD_HIP(hipMemcpyAsync(dOut, dP, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, s));
+  if (nt == 0) {
+    MLOD_HIP(hipStreamSynchronize(s));
+    return MLOD_OK;
+  }
+  const size_t ne = 3 * (size_t)nt;
+  MeshBufs B;
+  if (int r = mesh_sort(B, dTri, nt, 1, s)) return r;
+  MLOD_HIP(dalloc(&B.a, ne + 1));
+  MLOD_HIP(dalloc(&B.b, ne + 1));
+  MLOD_HIP(dalloc(&B.c, ne));
+  MLOD_HIP(dalloc(&B.f, ne + 1));
+  // heads: a = head index per sorted position, f = first-encounter flags (encounter order)
+  MLOD_HIP(hipMemsetAsync(B.f, 0, ne + 1, s));
+  k_edge_heads<<<blocks(ne), BS, 0, s>>>(B.skey, B.sval, (uint32_t)ne, B.a, B.f);
+  MLOD_HIP(hipGetLastError());
+  // b = the head of each sorted position's run (max-scan of the head indices); c = rank of each
+  // first encounter (exclusive scan of the flags)
+  size_t n1 = 0, n2 = 0;
+  MLOD_HIP(rocprim::inclusive_scan(nullptr, n1, B.a, B.b, ne, rocprim::maximum<uint32_t>(), s));
+  MLOD_HIP(rocprim::exclusive_scan(nullptr, n2, B.f, B.c, 0u, ne, rocprim::plus<uint32_t>(), s));
+  MLOD_HIP(hipMalloc(&B.tmp2, std::max(n1, n2) + 256));
+  size_t bytes = std::max(n1, n2);
+  MLOD_HIP(rocprim::inclusive_scan(B.tmp2, bytes, B.a, B.b, ne, rocprim::maximum<uint32_t>(), s));
+  bytes = std::max(n1, n2);
+  MLOD_HIP(rocprim::exclusive_scan(B.tmp2, bytes, B.f, B.c, 0u, ne, rocprim::plus<uint32_t>(), s));
+  uint32_t lastRank = 0;
+  uint8_t lastFlag = 0;
+  MLOD_HIP(hipMemcpyAsync(&lastRank, B.c + ne - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  MLOD_HIP(hipMemcpyAsync(&lastFlag, B.f + ne - 1, 1, hipMemcpyDeviceToHost, s));
+  MLOD_HIP(hipStreamSynchronize(s));
+  const uint32_t nEdges = lastRank + lastFlag;
+  // B.val is free now: the new vertex of every encounter
+  k_edge_verts<<<blocks(ne), BS, 0, s>>>(B.skey, B.sval, B.b, B.c, (uint32_t)ne, nv, dP, B.val, dOut);
+  MLOD_HIP(hipGetLastError());
+  k_subdiv_tris<<<blocks(nt), BS, 0, s>>>(dTri, nt, B.val, dOutTri);
+  MLOD_HIP(hipGetLastError());
+  MLOD_HIP(hipStreamSynchronize(s));
+  *outNv = nv + nEdges;
+  return MLOD_OK;
+}
+
+// the first n device points dV projected `steps` steps on t (its kernel, settings and mode,
+// default step bound) in place, normals into dN; the tree's uploaded queries are set aside
+int project_device(mlod_tree* t, float* dV, float* dN, uint32_t n, uint32_t steps, hipStream_t s) {
+  if (!n) return MLOD_OK;
+  const QueryState saved = t->qs;
+  t->qs = QueryState{};
+  int r = queries_set(t, dV, n, true, s);
+  if (!r && steps)
+    r = (t->mode == MLOD_MODE_KNN || t->mode == MLOD_MODE_BALL) ? launch_neighbors(t, n, steps, s)
+                                                                : launch_project(t, n, steps, t->radius, s);
+  if (!r) r = queries_scatter(t, n, dV, dN, s);
+  (void)hipStreamSynchronize(s);
+  queries_free(t);
+  t->qs = saved;
+  return r;
 }
 }  // namespace
 
@@ -1042,25 +1126,24 @@ int mlod_mesh_smooth(const float* verts, uint32_t nv, const uint32_t* tris, uint
   if (int r = check_tris(tris, nt, nv)) return r;
   if (nv == 0) return MLOD_OK;
   hipStream_t s = (hipStream_t)stream;
-  MeshBufs B;
-  const size_t ne = 3 * (size_t)nt;
-  MLOD_HIP(dalloc(&B.P, 3 * (size_t)nv));
-  MLOD_HIP(dalloc(&B.out, 3 * (size_t)nv));
-  MLOD_HIP(dalloc(&B.tri, ne));
-  MLOD_HIP(dalloc(&B.key, ne));
-  MLOD_HIP(dalloc(&B.skey, ne));
-  MLOD_HIP(dalloc(&B.val, ne));
-  MLOD_HIP(dalloc(&B.sval, ne));
-  MLOD_HIP(hipMemcpyAsync(B.P, verts, 3 * (size_t)nv * sizeof(float), hipMemcpyHostToDevice, s));
-  if (nt) {
-    MLOD_HIP(hipMemcpyAsync(B.tri, tris, ne * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    if (int r = mesh_sort(B, nt, 0, s)) return r;
+  DVec<float> P, out;
+  DVec<uint32_t> tri;
+  int r = P.reserve(3 * (size_t)nv, 0, s);
+  if (!r) r = out.reserve(3 * (size_t)nv, 0, s);
+  if (!r) r = tri.reserve(3 * (size_t)nt, 0, s);
+  hipError_t e = hipSuccess;
+  if (!r) e = hipMemcpyAsync(P.p, verts, 3 * (size_t)nv * sizeof(float), hipMemcpyHostToDevice, s);
+  if (!r && !e && nt) e = hipMemcpyAsync(tri.p, tris, 3 * (size_t)nt * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+  if (!r && e) r = hip_fail(e, "mesh smoothing", __FILE__, __LINE__);
+  if (!r) r = mesh_smooth_dev(P.p, nv, tri.p, nt, out.p, s);
+  if (!r) {
+    e = hipMemcpyAsync(out_verts, out.p, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (!e) e = hipStreamSynchronize(s);
+    if (e) r = hip_fail(e, "mesh smoothing", __FILE__, __LINE__);
   }
-  k_smooth<<<blocks(nv), BS, 0, s>>>(B.P, nv, B.tri, B.skey, B.sval, (uint32_t)ne, B.out);
-  MLOD_HIP(hipGetLastError());
-  MLOD_HIP(hipMemcpyAsync(out_verts, B.out, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToHost, s));
-  MLOD_HIP(hipStreamSynchronize(s));
-  return MLOD_OK;
+  (void)hipStreamSynchronize(s);
+  P.release(), out.release(), tri.release();
+  return r;
 }
 
 int mlod_mesh_subdivide(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* out_verts,
@@ -1074,56 +1157,91 @@ int mlod_mesh_subdivide(const float* verts, uint32_t nv, const uint32_t* tris, u
   std::memcpy(out_verts, verts, 3 * (size_t)nv * sizeof(float));
   *out_nv = nv;
   if (nt == 0) return MLOD_OK;
-  MeshBufs B;
-  const size_t ne = 3 * (size_t)nt;
-  MLOD_HIP(dalloc(&B.P, 3 * (size_t)nv));
-  MLOD_HIP(dalloc(&B.out, 3 * ((size_t)nv + ne)));
-  MLOD_HIP(dalloc(&B.tri, ne));
-  MLOD_HIP(dalloc(&B.otri, 4 * ne));
-  MLOD_HIP(dalloc(&B.key, ne));
-  MLOD_HIP(dalloc(&B.skey, ne));
-  MLOD_HIP(dalloc(&B.val, ne));
-  MLOD_HIP(dalloc(&B.sval, ne));
-  MLOD_HIP(dalloc(&B.a, ne + 1));
-  MLOD_HIP(dalloc(&B.b, ne + 1));
-  MLOD_HIP(dalloc(&B.c, ne));
-  MLOD_HIP(dalloc(&B.f, ne + 1));
-  MLOD_HIP(hipMemcpyAsync(B.P, verts, 3 * (size_t)nv * sizeof(float), hipMemcpyHostToDevice, s));
-  MLOD_HIP(hipMemcpyAsync(B.tri, tris, ne * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  if (int r = mesh_sort(B, nt, 1, s)) return r;
-  // heads: a = head index per sorted position, f = first-encounter flags (encounter order)
-  MLOD_HIP(hipMemsetAsync(B.f, 0, ne + 1, s));
-  k_edge_heads<<<blocks(ne), BS, 0, s>>>(B.skey, B.sval, (uint32_t)ne, B.a, B.f);
-  MLOD_HIP(hipGetLastError());
-  // b = the head of each sorted position's run (max-scan of the head indices); c = rank of each
-  // first encounter (exclusive scan of the flags, ne + 1 entries: the last is the edge count)
-  size_t n1 = 0, n2 = 0;
-  MLOD_HIP(rocprim::inclusive_scan(nullptr, n1, B.a, B.b, ne, rocprim::maximum<uint32_t>(), s));
-  MLOD_HIP(rocprim::exclusive_scan(nullptr, n2, B.f, B.c, 0u, ne, rocprim::plus<uint32_t>(), s));
-  void* t2 = nullptr;
-  MLOD_HIP(hipMalloc(&t2, std::max(n1, n2) + 256));
-  size_t bytes = std::max(n1, n2);
-  hipError_t e = rocprim::inclusive_scan(t2, bytes, B.a, B.b, ne, rocprim::maximum<uint32_t>(), s);
-  bytes = std::max(n1, n2);
-  if (!e) e = rocprim::exclusive_scan(t2, bytes, B.f, B.c, 0u, ne, rocprim::plus<uint32_t>(), s);
-  uint32_t lastRank = 0;
-  uint8_t lastFlag = 0;
-  if (!e) e = hipMemcpyAsync(&lastRank, B.c + ne - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
-  if (!e) e = hipMemcpyAsync(&lastFlag, B.f + ne - 1, 1, hipMemcpyDeviceToHost, s);
-  if (!e) e = hipStreamSynchronize(s);
-  (void)hipFree(t2);
-  if (e) return hip_fail(e, "mesh subdivision", __FILE__, __LINE__);
-  const uint32_t nEdges = lastRank + lastFlag;
-  // B.val is free now: the new vertex of every encounter
-  MLOD_HIP(hipMemcpyAsync(B.out, B.P, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, s));
-  k_edge_verts<<<blocks(ne), BS, 0, s>>>(B.skey, B.sval, B.b, B.c, (uint32_t)ne, nv, B.P, B.val, B.out);
-  MLOD_HIP(hipGetLastError());
-  k_subdiv_tris<<<blocks(nt), BS, 0, s>>>(B.tri, nt, B.val, B.otri);
-  MLOD_HIP(hipGetLastError());
-  MLOD_HIP(hipMemcpyAsync(out_verts, B.out, 3 * ((size_t)nv + nEdges) * sizeof(float), hipMemcpyDeviceToHost, s));
-  MLOD_HIP(hipMemcpyAsync(out_tris, B.otri, 4 * ne * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  DVec<float> P, out;
+  DVec<uint32_t> tri, otri;
+  int r = P.reserve(3 * (size_t)nv, 0, s);
+  if (!r) r = out.reserve(3 * ((size_t)nv + 3 * (size_t)nt), 0, s);
+  if (!r) r = tri.reserve(3 * (size_t)nt, 0, s);
+  if (!r) r = otri.reserve(12 * (size_t)nt, 0, s);
+  hipError_t e = hipSuccess;
+  if (!r) e = hipMemcpyAsync(P.p, verts, 3 * (size_t)nv * sizeof(float), hipMemcpyHostToDevice, s);
+  if (!r && !e) e = hipMemcpyAsync(tri.p, tris, 3 * (size_t)nt * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+  if (!r && e) r = hip_fail(e, "mesh subdivision", __FILE__, __LINE__);
+  uint32_t nOut = nv;
+  if (!r) r = mesh_subdivide_dev(P.p, nv, tri.p, nt, out.p, otri.p, &nOut, s);
+  if (!r) {
+    e = hipMemcpyAsync(out_verts, out.p, 3 * (size_t)nOut * sizeof(float), hipMemcpyDeviceToHost, s);
+    if (!e) e = hipMemcpyAsync(out_tris, otri.p, 12 * (size_t)nt * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (!e) e = hipStreamSynchronize(s);
+    if (e) r = hip_fail(e, "mesh subdivision", __FILE__, __LINE__);
+  }
+  (void)hipStreamSynchronize(s);
+  P.release(), out.release(), tri.release(), otri.release();
+  if (!r) *out_nv = nOut;
+  return r;
+}
+
+int mlod_dc_refine(mlod_dc* dc, mlod_tree* t, uint32_t steps, uint32_t rounds, uint32_t* n_verts, uint32_t* n_tris,
+                   void* stream) {
+  if (!dc || !t) return fail(MLOD_ERR_INVALID, "NULL argument");
+  if (!dc->extracted) return fail(MLOD_ERR_STATE, "no mesh (mlod_dc_extract)");
+  if (steps && !t->kernel_set) return fail(MLOD_ERR_STATE, "no kernel set (mlod_set_kernel)");
+  hipStream_t s = (hipStream_t)stream;
+  dc->refined = false;
+  // sizes after the rounds: vertices grow by at most 3 per triangle, triangles 4x per round
+  uint64_t nt = dc->nTris, nvMax = dc->nVerts;
+  for (uint32_t r = 0; r < rounds && nt; ++r) nvMax += 3 * nt, nt *= 4;
+  if (nt > 0x15555555ull || nvMax > 0xffffffffull) return fail(MLOD_ERR_UNSUPPORTED, "mesh: too many triangles");
+  DVec<float>& V = dc->rv;
+  DVec<float>& N = dc->rn;
+  DVec<uint32_t>& F = dc->rt;
+  DVec<float> V2;
+  DVec<uint32_t> F2;
+  int r = V.reserve(3 * (size_t)nvMax, 0, s);
+  if (!r) r = N.reserve(3 * (size_t)nvMax, 0, s);
+  if (!r) r = F.reserve(3 * (size_t)nt, 0, s);
+  if (!r) r = V2.reserve(3 * (size_t)nvMax, 0, s);
+  if (!r) r = F2.reserve(3 * (size_t)nt, 0, s);
+  uint32_t nv = dc->nVerts, ntri = dc->nTris;
+  if (!r && nv) {
+    hipError_t e = hipMemcpyAsync(V.p, dc->vframe.p, 3 * (size_t)nv * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (!e && ntri) e = hipMemcpyAsync(F.p, dc->u32.p, 3 * (size_t)ntri * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (!e) e = hipMemsetAsync(N.p, 0, 3 * (size_t)nv * sizeof(float), s);
+    if (e) r = hip_fail(e, "mesh refinement", __FILE__, __LINE__);
+  }
+  // scene.cpp:1417-1452: the vertices projected, then per round smoothing, subdivision and
+  // projection again
+  if (!r && steps) r = project_device(t, V.p, N.p, nv, steps, s);
+  for (uint32_t k = 0; !r && k < rounds && ntri; ++k) {
+    r = mesh_smooth_dev(V.p, nv, F.p, ntri, V2.p, s);
+    uint32_t nv2 = nv;
+    if (!r) r = mesh_subdivide_dev(V2.p, nv, F.p, ntri, V.p, F2.p, &nv2, s);
+    if (!r) {
+      std::swap(F, F2);
+      nv = nv2, ntri *= 4;
+      const hipError_t e = hipMemsetAsync(N.p, 0, 3 * (size_t)nv * sizeof(float), s);
+      if (e) r = hip_fail(e, "mesh refinement", __FILE__, __LINE__);
+    }
+    if (!r && steps) r = project_device(t, V.p, N.p, nv, steps, s);
+  }
+  (void)hipStreamSynchronize(s);
+  V2.release(), F2.release();
+  if (r) return r;
+  dc->nRV = nv, dc->nRT = ntri, dc->refined = true;
+  if (n_verts) *n_verts = nv;
+  if (n_tris) *n_tris = ntri;
+  return MLOD_OK;
+}
+
+int mlod_dc_refined_mesh(const mlod_dc* dc, float* verts, float* normals, uint32_t* tris, void* stream) {
+  if (!dc) return fail(MLOD_ERR_INVALID, "NULL argument");
+  if (!dc->refined) return fail(MLOD_ERR_STATE, "no refined mesh (mlod_dc_refine)");
+  hipStream_t s = (hipStream_t)stream;
+  if (verts && dc->nRV) MLOD_HIP(hipMemcpyAsync(verts, dc->rv.p, 3 * (size_t)dc->nRV * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (normals && dc->nRV)
+    MLOD_HIP(hipMemcpyAsync(normals, dc->rn.p, 3 * (size_t)dc->nRV * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (tris && dc->nRT) MLOD_HIP(hipMemcpyAsync(tris, dc->rt.p, 3 * (size_t)dc->nRT * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   MLOD_HIP(hipStreamSynchronize(s));
-  *out_nv = nv + nEdges;
   return MLOD_OK;
 }
 

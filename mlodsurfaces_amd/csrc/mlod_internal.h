@@ -209,6 +209,7 @@ int launch_neighbors(mlod_tree* t, uint32_t n, uint32_t n_iter, hipStream_t s);
 // queries.hip
 int queries_set(mlod_tree* t, const float* xyz, uint32_t m, bool device_src, hipStream_t s);
 int queries_get(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, hipStream_t s);
+int queries_scatter(mlod_tree* t, uint32_t n, float* d_pts, float* d_nmls, hipStream_t s);
 void queries_free(mlod_tree* t);
 int queries_reorganize(mlod_tree* t, const uint32_t* invalid, uint32_t n, uint32_t end_valid, hipStream_t s);
 int queries_fit_kinds(mlod_tree* t, uint32_t n, uint8_t* out, hipStream_t s);

@@ -184,6 +184,21 @@ int queries_get(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, hipSt
   return MLOD_OK;
 }
 
+// positions and normals of the first n queries in the caller's order into device arrays
+int queries_scatter(mlod_tree* t, uint32_t n, float* d_pts, float* d_nmls, hipStream_t s) {
+  QueryState& qs = t->qs;
+  const uint32_t nb = (qs.m + BS - 1) / BS;
+  if (d_pts && qs.m) {
+    k_scatter<<<nb, BS, 0, s>>>(qs.q, qs.perm, qs.m, n, d_pts);
+    MLOD_HIP(hipGetLastError());
+  }
+  if (d_nmls && qs.m) {
+    k_scatter<<<nb, BS, 0, s>>>(qs.qn, qs.perm, qs.m, n, d_nmls);
+    MLOD_HIP(hipGetLastError());
+  }
+  return MLOD_OK;
+}
+
 }  // namespace mlod
 
 namespace mlod {

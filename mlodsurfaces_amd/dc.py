@@ -92,6 +92,12 @@ class DualContouring:
         u = (2 * L + 1).astype(np.float32) * np.float32(2.0 ** -(d + 1))
         return (m00 * u + m03).astype(np.float32)
 
+    def refine(self, apss, steps=3, rounds=2):
+        """The extracted mesh kept on the device: vertices projected `steps` steps with `apss`'s
+        tree and settings, then `rounds` times smoothing, subdivision and projection
+        (scene.cpp:1417-1452); returns (vertices, normals, triangles) in the projection frame."""
+        return _refine(self, apss, steps, rounds)
+
     def extract(self):
         """(vertices in the projection frame, triangles, node of each vertex, unit-frame vertices)"""
         nv, nt = C.c_uint32(), C.c_uint32()
@@ -102,6 +108,18 @@ class DualContouring:
         t = np.zeros((nt.value, 3), np.uint32)
         check(lib().mlod_dc_mesh(self.h, _p(v), _p(u), _p(vn), _p(t), self._stream))
         return v, t, vn, u
+
+
+def _refine(dc, apss, steps, rounds):
+    apss._sync_params()
+    nv, nt = C.c_uint32(), C.c_uint32()
+    check(lib().mlod_dc_refine(dc.h, apss.m_apssOctree.handle, int(steps), int(rounds), C.byref(nv), C.byref(nt),
+                               dc._stream))
+    V = np.zeros((nv.value, 3), np.float32)
+    N = np.zeros((nv.value, 3), np.float32)
+    F = np.zeros((nt.value, 3), np.uint32)
+    check(lib().mlod_dc_refined_mesh(dc.h, _p(V), _p(N), _p(F), dc._stream))
+    return V, N, F
 
 
 def grid_centres(apss, depth, cap=None, seed=0):
@@ -165,20 +183,13 @@ def surface_mesh(apss, depth, cut=False, reproject_steps=3, refine=2, log=None):
     apss._sync_params()
     dc.project_corners(apss.m_apssOctree)
     t3 = time.perf_counter()
-    V, F, VN, U = dc.extract()
+    nv, nt = C.c_uint32(), C.c_uint32()
+    check(lib().mlod_dc_extract(dc.h, C.byref(nv), C.byref(nt), dc._stream))
     t4 = time.perf_counter()
-    N = np.zeros_like(V)
-    if reproject_steps and len(V):
-        V, N = apss.project_points(V, reproject_steps)
-    for _ in range(refine):
-        if not len(F):
-            break
-        V = smooth(V, F)
-        V, F = subdivide(V, F)
-        V, N = apss.project_points(V, reproject_steps)
+    V, N, F = dc.refine(apss, reproject_steps, refine)
     t5 = time.perf_counter()
-    info = dict(cells=len(cells), nodes=inf["nodes"], leaves=inf["leaves"], corners=inf["corners"], dc_vertices=len(VN),
-                dc_triangles=int(len(F) // 4 ** refine) if len(F) else 0, vertices=len(V), triangles=len(F),
+    info = dict(cells=len(cells), nodes=inf["nodes"], leaves=inf["leaves"], corners=inf["corners"],
+                dc_vertices=nv.value, dc_triangles=nt.value, vertices=len(V), triangles=len(F),
                 ms_generate=(t1 - t0) * 1e3, ms_uniformize=(t2 - t1) * 1e3, ms_corners=(t3 - t2) * 1e3,
                 ms_extract=(t4 - t3) * 1e3, ms_refine=(t5 - t4) * 1e3)
     if log:

@@ -150,6 +150,15 @@ def test_projection_field_pipeline(oracle_built):
     P, T = subdivide(Vs, F)
     oP, oT = oracle_built.oracle_mesh_subdivide(Vs, F)
     assert np.array_equal(P, oP) and np.array_equal(T, oT)
+    # the device-resident refinement = the host passes with the host projection, bit for bit
+    Vh, Nh = ap.project_points(V, 3)
+    Fh = F
+    for _ in range(2):
+        Vh = smooth(Vh, Fh)
+        Vh, Fh = subdivide(Vh, Fh)
+        Vh, Nh = ap.project_points(Vh, 3)
+    Vd, Nd, Fd = dc.refine(ap, 3, 2)
+    assert np.array_equal(Vd, Vh) and np.array_equal(Nd, Nh) and np.array_equal(Fd, Fh)
     # the whole pipeline: vertices projected three steps lie on the surface, twice refined
     V3, N3, F3, info = surface_mesh(ap, 6, refine=0)
     assert info["triangles"] == len(F) and info["vertices"] == len(V)
