@@ -240,6 +240,11 @@ typedef struct mlod_dc mlod_dc;
 int mlod_dc_create(mlod_dc** out);
 int mlod_dc_destroy(mlod_dc* dc);
 int mlod_dc_build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t max_depth, int32_t cut, void* stream);
+/* uniformizeFromPoints(cells, ...) (upsample.cpp:253-259), the variant scene::computeDualContouring
+ * takes for an octree built on points (scene.cpp:1279-1299: OctreeGenerator::generateFromPoints
+ * cells, mlod_generate_from_points): grade, regularize up and siblings, in the [-1, 1]^3 frame
+ * (no re-rooting, no cut). */
+int mlod_dc_build_from_points(mlod_dc* dc, const int32_t* ids4, uint32_t n, void* stream);
 /* [nodes, leaves, corners, max depth, diag, origin x, origin y, origin z, extracted] */
 int mlod_dc_info(const mlod_dc* dc, double* out9);
 /* the uniformized octree in node_comparator order: ids4 (n x 4) and leaf flags (either may be NULL) */

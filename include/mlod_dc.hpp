@@ -10,7 +10,8 @@
 //   bit, on the GPU.
 // * mlod::computeDualContouring — the reference's whole sequence on the adaptive octree:
 //   OctreeGenerator::generate with its crossed cells, uniformize, corners, extraction, the
-//   vertices projected 3 steps, then twice smoothing, subdivision and projection.
+//   vertices projected 3 steps, then twice smoothing, subdivision and projection;
+//   computeDualContouringOnPoints — the same on an octree built on points (uniformizeFromPoints).
 // * mlod::meshComments — the kernel and octree comment lines the reference writes into the mesh
 //   file (scene.cpp:1498-1530); Pn::saveMesh (mlod_pn.hpp) writes the file.
 //
@@ -31,13 +32,17 @@ namespace mlod {
 
 class DualContouring {
  public:
-  // uniformize(max_depth, cells, ..., cut) of the cells (usually OctreeGenerator's crossed cells)
-  DualContouring(const std::vector<octree_id>& cells, unsigned max_depth = 0, bool cut = false, void* stream = nullptr)
+  // uniformize(max_depth, cells, ..., cut) of the cells (usually OctreeGenerator's crossed cells),
+  // or with fromPoints uniformizeFromPoints(cells) (upsample.cpp:253-259: cells of an octree
+  // built on points by OctreeGenerator::generateFromPoints; no re-rooting, no cut)
+  DualContouring(const std::vector<octree_id>& cells, unsigned max_depth = 0, bool cut = false, void* stream = nullptr,
+                 bool fromPoints = false)
       : m_stream(stream) {
     check(mlod_dc_create(&m_dc));
     static_assert(sizeof(octree_id) == 4 * sizeof(int32_t), "octree_id is four int32");
-    const int r = mlod_dc_build(m_dc, reinterpret_cast<const int32_t*>(cells.data()), (uint32_t)cells.size(), max_depth,
-                                cut ? 1 : 0, m_stream);
+    const int32_t* ids = reinterpret_cast<const int32_t*>(cells.data());
+    const int r = fromPoints ? mlod_dc_build_from_points(m_dc, ids, (uint32_t)cells.size(), m_stream)
+                             : mlod_dc_build(m_dc, ids, (uint32_t)cells.size(), max_depth, cut ? 1 : 0, m_stream);
     if (r != MLOD_OK) {
       mlod_dc_destroy(m_dc);
       check(r);
@@ -133,9 +138,28 @@ inline void computeDualContouring(const APSS& apss, const Kernel& kernel, unsign
   dc.refine(apss, reprojectSteps, refine, vertices, normals, indices);
 }
 
+// scene::computeDualContouring on an octree built on points (scene.cpp:1279-1299: the usual
+// octree on the input points, or the octree on the projected points): the cells of
+// OctreeGenerator::generateFromPoints(maxDepth, points), uniformizeFromPoints, then as above.
+template <class V>
+inline void computeDualContouringOnPoints(const APSS& apss, const Kernel& kernel, unsigned maxDepth,
+                                          const std::vector<V>& points, std::vector<Vec3f>& vertices,
+                                          std::vector<Vec3f>& normals, std::vector<uint32_t>& indices,
+                                          unsigned reprojectSteps = 3, unsigned refine = 2) {
+  std::vector<octree_id> cells;
+  {
+    OctreeGenerator gen(const_cast<APSS*>(&apss), kernel);
+    cells = gen.generateFromPoints(maxDepth, points);
+  }
+  DualContouring dc(cells, maxDepth, false, nullptr, true);
+  dc.projectCorners(apss);
+  dc.extract(vertices, indices);
+  dc.refine(apss, reprojectSteps, refine, vertices, normals, indices);
+}
+
 // The comment lines of the reference's mesh file (scene.cpp:1498-1530) for a kernel whose sigmas
 // were scaled by a cloud of bounding-box radius `radius`, on the adaptive octree.
-inline std::vector<std::string> meshComments(const Kernel& k, float radius) {
+inline std::vector<std::string> meshComments(const Kernel& k, float radius, bool adaptiveOctree = true) {
   std::vector<std::string> c{"Created by mlodsurfaces_amd", "Kernel parameters:"};
   auto line = [&c](const std::string& head, auto v) {
     std::ostringstream o;
@@ -164,7 +188,7 @@ inline std::vector<std::string> meshComments(const Kernel& k, float radius) {
       c.push_back("Kernel unknown");
   }
   c.push_back("Octree:");
-  c.push_back("Built using the adaptive octree method");
+  c.push_back(adaptiveOctree ? "Built using the adaptive octree method" : "Built on input points");
   return c;
 }
 

@@ -85,6 +85,7 @@ SIGNATURES = {
     "mlod_dc_create": (C.c_int, [C.POINTER(_vp)]),
     "mlod_dc_destroy": (C.c_int, [_vp]),
     "mlod_dc_build": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_int32, _vp]),
+    "mlod_dc_build_from_points": (C.c_int, [_vp, _vp, C.c_uint32, _vp]),
     "mlod_dc_info": (C.c_int, [_vp, _vp]),
     "mlod_dc_nodes": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mlod_dc_corners": (C.c_int, [_vp, _vp, _vp, _vp]),

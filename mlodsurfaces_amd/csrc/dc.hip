@@ -628,7 +628,9 @@ void frame_matrix(const mlod_dc* dc, float4& m00, float4& m03) {
   m03 = make_float4(two * dc->origin[0] + -1.0f, two * dc->origin[1] + -1.0f, two * dc->origin[2] + -1.0f, 0.f);
 }
 
-int build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t maxDepth, bool cut, hipStream_t s) {
+// uniformize(maxDepth, cells, ..., cut) or, with fromPoints, uniformizeFromPoints(cells)
+// (upsample.cpp:253-259: no depth-0 ancestors, no re-rooting, no cut)
+int build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t maxDepth, bool cut, bool fromPoints, hipStream_t s) {
   dc->extracted = dc->haveValues = dc->refined = false;
   dc->nLeaves = dc->nVerts = dc->nTris = 0;
   MLOD_HIP(hipMemsetAsync(dc->dFlags, 0, sizeof(uint32_t), s));
@@ -646,12 +648,12 @@ int build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t maxDepth, bool 
   if (e) return hip_fail(e, "dual contouring: cells", __FILE__, __LINE__);
   if (int r = check_flags(dc, s)) return r;
   // depth-0 ancestors appended (upsample.cpp:264)
-  if (n) {
+  if (n && !fromPoints) {
     k_ancestor0<<<blocks(n), BS, 0, s>>>(dc->keys.p, n);
     MLOD_HIP(hipGetLastError());
   }
   size_t m = 0;
-  if (int r = sort_unique(dc, dc->keys, 2 * (size_t)n, &m, s)) return r;
+  if (int r = sort_unique(dc, dc->keys, (fromPoints ? 1 : 2) * (size_t)n, &m, s)) return r;
   // retreiveCorrectRoot (upsample.cpp:401-406) from the depth-0 cells
   uint32_t b[17];
   if (int r = depth_bounds(dc, b, s)) return r;
@@ -668,6 +670,7 @@ int build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t maxDepth, bool 
     for (int a = 0; a < 3; ++a) delta = std::max(delta, (int)std::ceil(float(std::log2(mx[a] - mn[a] + 1))));
   dc->diag = 1.f;
   dc->origin[0] = dc->origin[1] = dc->origin[2] = 0.f;
+  if (fromPoints) delta = 0, cut = false;
   if (!d0.empty() && delta > 0) {
     k_shift<<<blocks(dc->keys.n), BS, 0, s>>>(dc->keys.p, (uint32_t)dc->keys.n, mn[0], mn[1], mn[2], delta, dc->dFlags);
     MLOD_HIP(hipGetLastError());
@@ -866,7 +869,13 @@ int mlod_dc_destroy(mlod_dc* dc) {
 int mlod_dc_build(mlod_dc* dc, const int32_t* ids4, uint32_t n, uint32_t max_depth, int32_t cut, void* stream) {
   if (!dc || !ids4) return fail(MLOD_ERR_INVALID, "NULL argument");
   if (n == 0) return fail(MLOD_ERR_INVALID, "dual contouring: no cells");
-  return build(dc, ids4, n, max_depth, cut != 0, (hipStream_t)stream);
+  return build(dc, ids4, n, max_depth, cut != 0, false, (hipStream_t)stream);
+}
+
+int mlod_dc_build_from_points(mlod_dc* dc, const int32_t* ids4, uint32_t n, void* stream) {
+  if (!dc || !ids4) return fail(MLOD_ERR_INVALID, "NULL argument");
+  if (n == 0) return fail(MLOD_ERR_INVALID, "dual contouring: no cells");
+  return build(dc, ids4, n, 0, false, true, (hipStream_t)stream);
 }
 
 int mlod_dc_info(const mlod_dc* dc, double* out9) {

@@ -23,13 +23,20 @@ def _p(a):
 
 
 class DualContouring:
-    def __init__(self, cells, max_depth=0, cut=False, stream=None):
+    """uniformize(max_depth, cells, ..., cut) of octree cells (upsample.cpp:262-272), or with
+    from_points uniformizeFromPoints(cells) (253-259: cells of an octree built on points, no
+    re-rooting, no cut); then the corner field and the extraction."""
+
+    def __init__(self, cells, max_depth=0, cut=False, stream=None, from_points=False):
         ids = np.ascontiguousarray(cells, np.int32).reshape(-1, 4)
         h = C.c_void_p()
         check(lib().mlod_dc_create(C.byref(h)))
         self.h = h
         self._stream = stream
-        check(lib().mlod_dc_build(self.h, _p(ids), len(ids), int(max_depth), int(bool(cut)), stream))
+        if from_points:
+            check(lib().mlod_dc_build_from_points(self.h, _p(ids), len(ids), stream))
+        else:
+            check(lib().mlod_dc_build(self.h, _p(ids), len(ids), int(max_depth), int(bool(cut)), stream))
 
     def close(self):
         if getattr(self, "h", None):
@@ -163,21 +170,28 @@ def subdivide(V, F):
     return out[:n.value], T
 
 
-def surface_mesh(apss, depth, cut=False, reproject_steps=3, refine=2, log=None):
+def surface_mesh(apss, depth, cut=False, reproject_steps=3, refine=2, log=None, points=None):
     """The reference's mesh of the surface for an APSS plugin `apss` (Fastapss or GlobalAPSS with
     its kernel and settings installed): OctreeGenerator.generate(depth) with its crossed cells,
     uniformize, corner field by projection, dual contouring, then the vertices projected
     `reproject_steps` steps, then `refine` rounds of smoothing, subdivision and projection
-    (scene.cpp:1417-1452).  Returns (vertices, normals, triangles, info)."""
+    (scene.cpp:1417-1452).  With `points` (the input points for the reference's usual octree, or
+    its projected points for the octree on upsampled points), the cells are those of
+    OctreeGenerator.generateFromPoints(depth, points), uniformized from points
+    (scene.cpp:1279-1299).  Returns (vertices, normals, triangles, info)."""
     import time
     from .apss import OctreeGenerator
     t0 = time.perf_counter()
     gen = OctreeGenerator(apss, apss.kernel if hasattr(apss, "kernel") else None)
-    total = []
-    gen.generate(depth, total)
-    cells = total[0]
+    if points is None:
+        total = []
+        gen.generate(depth, total)
+        cells = total[0]
+    else:
+        cells = gen.generateFromPoints(depth, points)
+    gen.close()
     t1 = time.perf_counter()
-    dc = DualContouring(cells, depth, cut)
+    dc = DualContouring(cells, depth, cut, from_points=points is not None)
     inf = dc.info()
     t2 = time.perf_counter()
     apss._sync_params()

@@ -219,10 +219,15 @@ def write_mesh(path: str, V: np.ndarray, N: np.ndarray, F: np.ndarray, frame: Fr
         f.write(face.tobytes())
 
 
-def mesh_comments(kernel, radius: float) -> list:
+_OCTREE_LINES = {"adaptive": "Built using the adaptive octree method", "points": "Built on input points",
+                 "upsampled": "Built on upsampled points"}
+
+
+def mesh_comments(kernel, radius: float, octree: str = "adaptive") -> list:
     """The comment lines the reference writes into its mesh file (scene.cpp:1498-1530): the
     kernel's parameters (as C++ streams print floats: six significant digits) and the octree
-    method, for a kernel whose sigmas were scaled by a cloud of bounding-box radius `radius`."""
+    method ("adaptive", "points" for the octree on the input points, "upsampled"), for a kernel
+    whose sigmas were scaled by a cloud of bounding-box radius `radius`."""
     f32 = np.float32
     g = lambda v: format(float(f32(v)), "g")  # noqa: E731
     c = ["Created by mlodsurfaces_amd", "Kernel parameters:"]
@@ -235,7 +240,7 @@ def mesh_comments(kernel, radius: float) -> list:
         c += ["Gaussian", f"Sigma {g(f32(kernel.sigma) / f32(radius))}", f"Constant_shift {g(kernel.constant_shift)}"]
     else:
         c += ["Kernel unknown"]
-    return c + ["Octree:", "Built using the adaptive octree method"]
+    return c + ["Octree:", _OCTREE_LINES[octree]]
 
 
 def read_mesh(path: str):
@@ -338,12 +343,13 @@ def write_timings(path: str, model: str, n_input: int, n_projected: int, step_ms
 
 
 def project_model(model: str, params_path: str | None = None, out: str | None = None, precision="fp32c",
-                  semantics="ref_gpu", log=print, mesh_depth: int | None = None):
+                  semantics="ref_gpu", log=print, mesh_depth: int | None = None, mesh_octree: str = "adaptive"):
     """The reference's automatic projection of a model's own points (mainwindow.cpp:180-220,
     scene.cpp:916-966) with its .params settings: read, normalise, build the octree, project the
     points maxNbIterations steps, write <model>Points.ply (file frame) and <model>.timings; with
     mesh_depth (0: the .params maxDepth), also the dual-contouring surface mesh <model>Mesh.ply
-    (scene::computeDualContouring)."""
+    (scene::computeDualContouring) on the adaptive octree, or with mesh_octree="points" on the
+    octree built on the normalised input points (the reference's usual octree)."""
     from . import apss as A
     from . import hiprt
     P, N = read_points(model)
@@ -374,8 +380,9 @@ def project_model(model: str, params_path: str | None = None, out: str | None = 
     write_points(base + "Points.ply", op, on, frame)
     if mesh_depth is not None:
         from .dc import surface_mesh
-        V, N, F, info = surface_mesh(ap, mesh_depth if mesh_depth > 0 else prm.max_depth)
-        write_mesh(base + "Mesh.ply", V, N, F, frame, comments=mesh_comments(K, tree.radius()))
+        V, N, F, info = surface_mesh(ap, mesh_depth if mesh_depth > 0 else prm.max_depth,
+                                     points=Pn if mesh_octree == "points" else None)
+        write_mesh(base + "Mesh.ply", V, N, F, frame, comments=mesh_comments(K, tree.radius(), mesh_octree))
         log(f"{model}: surface mesh of {info['triangles']} triangles at depth {mesh_depth or prm.max_depth} -> "
             f"{base}Mesh.ply")
     write_timings(base + ".timings", model, len(P), len(Pn), steps, gpu=True,
@@ -393,10 +400,12 @@ def main(argv=None):
     ap.add_argument("--precision", default="fp32c", choices=["fp32", "fp32c", "fp64"])
     ap.add_argument("--semantics", default="ref_gpu", choices=["ref_gpu", "ref_cpu"])
     ap.add_argument("--mesh", type=int, default=None, metavar="DEPTH",
-                    help="also write the surface mesh at this adaptive depth (0: the .params maxDepth)")
+                    help="also write the surface mesh at this depth (0: the .params maxDepth)")
+    ap.add_argument("--mesh-octree", default="adaptive", choices=["adaptive", "points"],
+                    help="octree of the mesh: the adaptive octree, or the octree built on the input points")
     a = ap.parse_args(argv)
     project_model(a.model, a.params, a.out, a.precision, a.semantics, log=lambda m: print(m, file=sys.stderr),
-                  mesh_depth=a.mesh)
+                  mesh_depth=a.mesh, mesh_octree=a.mesh_octree)
 
 
 if __name__ == "__main__":

@@ -102,6 +102,8 @@ def oracle_lib():
         L.orc_gen_ids.argtypes = [_vp, C.c_int32, _i]
         L.orc_dc_build.restype = _vp
         L.orc_dc_build.argtypes = [_i, C.c_uint32, C.c_uint32, C.c_int32]
+        L.orc_dc_build_points.restype = _vp
+        L.orc_dc_build_points.argtypes = [_i, C.c_uint32]
         L.orc_dc_free.argtypes = [_vp]
         L.orc_dc_num_nodes.restype = C.c_uint32
         L.orc_dc_num_nodes.argtypes = [_vp]
@@ -370,12 +372,16 @@ def oracle_cell_ancestor(id4, depth):
 
 class OracleDC:
     """Dual contouring of the oracle (dc_oracle.cpp): uniformize the given octree cells (ids4:
-    depth, x, y, z), set corner values at max-depth integer corner positions, extract."""
+    depth, x, y, z) — or uniformizeFromPoints them with from_points —, set corner values at
+    max-depth integer corner positions, extract."""
 
-    def __init__(self, ids4, max_depth=0, cut=False):
+    def __init__(self, ids4, max_depth=0, cut=False, from_points=False):
         self.L = oracle_lib()
         ids = np.ascontiguousarray(ids4, np.int32).reshape(-1, 4)
-        self.h = self.L.orc_dc_build(_ptr(ids, C.c_int32), len(ids), int(max_depth), int(bool(cut)))
+        if from_points:
+            self.h = self.L.orc_dc_build_points(_ptr(ids, C.c_int32), len(ids))
+        else:
+            self.h = self.L.orc_dc_build(_ptr(ids, C.c_int32), len(ids), int(max_depth), int(bool(cut)))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -4,7 +4,8 @@
 // the GPU implementation (mlodsurfaces_amd/csrc/dc.hip):
 //   * uniformize (sources/scene/upsample.cpp:262-272): depth0_ancestors (360-367),
 //     retreiveCorrectRoot / union_nodes (369-406), grade (67-93), regularize_up (40-46), cut
-//     (95-100), regularize_siblings (48-65), then sorted_octree (octree.h:232-312);
+//     (95-100), regularize_siblings (48-65), then sorted_octree (octree.h:232-312); and
+//     uniformizeFromPoints (253-259): grade, regularize_up, regularize_siblings only;
 //   * dual_contour::extract (sources/dual-contouring/dual_contouring.h:59-87) over
 //     minimal_edges::traverse (octree.h:400-537), with compute_vertex (130-176) and the corner
 //     values of scene::computeDualContouring (scene.cpp:1346-1396) given by the caller.
@@ -96,7 +97,11 @@ uint64_t corner_key(long x, long y, long z) {
   return ((uint64_t)(uint32_t)z << 42) | ((uint64_t)(uint32_t)y << 21) | (uint64_t)(uint32_t)x;
 }
 
-void uniformize(Tree& T, std::vector<Id> nodes, unsigned maxDepthCut, bool doCut) {
+void uniformize(Tree& T, std::vector<Id> nodes, unsigned maxDepthCut, bool doCut, bool fromPoints = false) {
+  if (fromPoints) {  // uniformizeFromPoints: no re-rooting, no cut
+    T.diag = 1.f;
+    doCut = false;
+  } else {
   // depth0_ancestors, appended
   std::vector<Id> d0(nodes.size());
   for (size_t i = 0; i < nodes.size(); ++i) d0[i] = ancestor0(nodes[i]);
@@ -124,6 +129,7 @@ void uniformize(Tree& T, std::vector<Id> nodes, unsigned maxDepthCut, bool doCut
   } else {
     // union_nodes returns depth -1: no shift; the reference's 1 << -1 is undefined, taken as 1
     T.diag = 1.f;
+  }
   }
   grade(nodes);
   // regularize_up (upsample.cpp:40-46)
@@ -317,6 +323,14 @@ void* orc_dc_build(const int32_t* ids4, uint32_t n, uint32_t max_depth, int32_t 
   for (uint32_t i = 0; i < n; ++i) nodes[i] = Id{ids4[4 * i], ids4[4 * i + 1], ids4[4 * i + 2], ids4[4 * i + 3]};
   Tree* T = new Tree();
   uniformize(*T, nodes, max_depth, cut != 0);
+  return T;
+}
+// uniformizeFromPoints (upsample.cpp:253-259)
+void* orc_dc_build_points(const int32_t* ids4, uint32_t n) {
+  std::vector<Id> nodes(n);
+  for (uint32_t i = 0; i < n; ++i) nodes[i] = Id{ids4[4 * i], ids4[4 * i + 1], ids4[4 * i + 2], ids4[4 * i + 3]};
+  Tree* T = new Tree();
+  uniformize(*T, nodes, 0, false, true);
   return T;
 }
 void orc_dc_free(void* h) { delete static_cast<Tree*>(h); }

@@ -199,3 +199,37 @@ def test_grid_centres(shift):
     assert {tuple(r) for r in q.tolist()} == {tuple(r) for r in expect.tolist()}
     capped = grid_centres(ap, 6, cap=100, seed=3)
     assert len(capped) == 100 and {tuple(r) for r in capped.tolist()} <= {tuple(r) for r in q.tolist()}
+
+
+def test_from_points_octree(oracle_built):
+    """uniformizeFromPoints on the GPU equals the oracle (nodes, leaves, mesh of an analytic
+    field), and surface_mesh on the octree built on the points (the reference's usual octree)
+    lies on the sphere."""
+    p, n = ds.sphere(30000, seed=9, noise=0.0)
+    p = (p * 0.6 + np.array([0.55, -0.1, 0.1])).astype(np.float32)  # reaches outside [-1, 1] in x
+    tree = A.APSSOctree().build(p, n)
+    k = A.Kernel(type=2, sigmas=[float(np.float32(0.01 * 2 ** i) * np.float32(tree.radius())) for i in range(4)])
+    ap = A.Fastapss(tree, k)
+    ap.setScalingProtectionSphere(2.0)
+    ap.set_precision("fp32c")
+    cells = A.OctreeGenerator(ap, k).generateFromPoints(6, p)
+    dc = DualContouring(cells, from_points=True)
+    nodes, leaf = dc.nodes()
+    o = oracle_built.OracleDC(cells, from_points=True)
+    on, ol = o.nodes()
+    keep = inside(on)
+    assert np.array_equal(nodes, on[keep]) and np.array_equal(leaf, ol[keep])
+    inf = dc.info()
+    assert inf["diag"] == 1.0 and np.array_equal(inf["origin"], np.zeros(3, np.float32)) and inf["max_depth"] == 6
+    xyz, pos = dc.corners()
+    f = (np.linalg.norm(pos.astype(np.float64) - [0.55, -0.1, 0.1], axis=1) - 0.6).astype(np.float32)
+    dc.set_corner_values(f)
+    o.set_corners(xyz, f)
+    gpu = dc.extract()
+    assert len(gpu[1]) > 1000
+    compare_meshes(gpu, o.extract(), nodes, on)
+    V, N, F, info = surface_mesh(ap, 6, points=p)
+    assert info["triangles"] == len(F) > 1000
+    r = np.linalg.norm(V.astype(np.float64) - [0.55, -0.1, 0.1], axis=1)
+    inside_root = np.all(np.abs(V) < 1, axis=1)
+    assert inside_root.mean() > 0.5 and np.abs(r[inside_root] - 0.6).max() < 1e-3

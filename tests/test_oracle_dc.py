@@ -227,3 +227,52 @@ def test_mesh_passes(oracle_built):
         assert np.array_equal(P[i], ((S[u] + S[w]) / np.float32(2)).astype(np.float32))
     dup, counts, ne = manifold(T)
     assert dup == 1 and counts == {2} and len(P) - ne + len(T) == 2
+
+
+def test_uniformize_from_points(oracle_built):
+    """uniformizeFromPoints (upsample.cpp:253-259): for cells inside the root it equals uniformize
+    (whose only depth-0 ancestor is then the root and whose re-rooting is the identity); for cells
+    reaching outside it equals the set pipeline without a shift (graded closure, ancestors, sibling
+    sets), in the [-1, 1]^3 frame (diag 1, origin 0)."""
+    d = 5
+    g = (np.arange(2 ** d) + 0.5) / 2 ** d
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    near = np.abs(np.linalg.norm(P - 0.5, axis=1) - 0.3) < 0.05
+    ids = np.concatenate([np.full((near.sum(), 1), d), np.floor(P[near] * 2 ** d).astype(int)], 1)
+    a, la = oracle_built.OracleDC(ids, from_points=True).nodes()
+    b, lb = oracle_built.OracleDC(ids).nodes()
+    assert np.array_equal(a, b) and np.array_equal(la, lb)
+    # reaching outside the root: no re-rooting
+    g = (np.arange(-4, 2 ** d + 4) + 0.5) / 2 ** d
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    near = np.abs(np.linalg.norm(P - 0.6, axis=1) - 0.5) < 0.04
+    ids = np.concatenate([np.full((near.sum(), 1), d), np.floor(P[near] * 2 ** d).astype(int)], 1)
+    dc = oracle_built.OracleDC(ids, from_points=True)
+    nodes, leaf = dc.nodes()
+    diag, origin, D = dc.frame()
+    assert diag == 1.0 and np.array_equal(origin, np.zeros(3, np.float32)) and D == d
+    G, todo = set(map(tuple, ids.tolist())), [tuple(r) for r in ids.tolist()]
+    while todo:
+        dd, x, y, z = todo.pop()
+        if dd == 0:
+            continue
+        for sx, sy, sz in itertools.product((-1, 1), repeat=3):
+            q = (dd - 1, (x + sx) >> 1, (y + sy) >> 1, (z + sz) >> 1)
+            if q not in G:
+                G.add(q)
+                todo.append(q)
+    A = set(G)
+    for n in G:
+        dd, x, y, z = n
+        while dd > 0:
+            dd, x, y, z = dd - 1, x >> 1, y >> 1, z >> 1
+            A.add((dd, x, y, z))
+    full = set(A) | {(0, 0, 0, 0)}
+    for dd, x, y, z in A:
+        if dd > 0:
+            for c in range(8):
+                full.add((dd, 2 * (x >> 1) + (c & 1), 2 * (y >> 1) + ((c >> 1) & 1), 2 * (z >> 1) + ((c >> 2) & 1)))
+    inside = lambda n: all(0 <= c < 2 ** n[0] for c in n[1:])  # noqa: E731
+    assert {n for n in map(tuple, nodes.tolist()) if inside(n)} == {n for n in full if inside(n)}

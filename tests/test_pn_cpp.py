@@ -86,6 +86,7 @@ def test_mesh_comments_kernel_types():
         ["Singular", "Exponent 4", "Constant_shift 0.001"]
     assert pointio.mesh_comments(Kernel(type=0, sigma=0.05), 0.5)[2:4] == ["Gaussian", "Sigma 0.1"]
     assert pointio.mesh_comments(Kernel(type=0), 1.0)[-2:] == ["Octree:", "Built using the adaptive octree method"]
+    assert pointio.mesh_comments(Kernel(type=0), 1.0, "points")[-1] == "Built on input points"
 
 
 @pytest.mark.gpu
