@@ -86,3 +86,22 @@ def test_project_model_files(tmp_path):
     print(f"mesh {len(V)} vertices {len(F)} triangles; sample-to-mesh distance / gap: median "
           f"{np.median(cover) / gap:.2f}, 90% {np.quantile(cover, 0.9) / gap:.2f}")
     assert np.median(cover) < 2 * gap and np.quantile(cover, 0.9) < 4 * gap
+
+
+def test_project_model_mesh_on_points(tmp_path):
+    """The mesh on the octree built on the model's own points (the reference's usual octree,
+    uniformizeFromPoints) also covers the scan, and its file says which octree it was built on."""
+    g = load_golden("real_Face.npz")
+    model = str(tmp_path / "Face.ply")
+    pointio.write_points(model, g["points"], g["normals"])
+    (tmp_path / "Face.params").write_bytes(g["params"].tobytes())
+    pointio.project_model(model, log=lambda m: None, mesh_depth=0, mesh_octree="points")
+    path = str(tmp_path / "FaceMesh.ply")
+    assert b"comment Built on input points\n" in open(path, "rb").read(2000)
+    V, N, F = pointio.read_mesh(path)
+    assert len(F) > 1000 and np.isfinite(V).all() and F.max() < len(V)
+    from scipy.spatial import cKDTree
+    gap = np.median(cKDTree(g["points"]).query(g["points"], k=2)[0][:, 1])
+    cover, _ = cKDTree(V).query(g["points"])
+    print(f"points-octree mesh {len(V)} vertices; sample-to-mesh / gap: median {np.median(cover) / gap:.2f}")
+    assert np.median(cover) < 2 * gap
