@@ -88,6 +88,44 @@ class Pn {
     for (Vec3f& p : positions) p = {(p.x - off.x) / norme, (p.y - off.y) / norme, (p.z - off.z) / norme};
   }
 
+  // Pn::order (Pn.cpp:388-445): the positions sorted along the Morton curve of a sizeOfGrid^3
+  // grid over the box [mini, maxi]^3 (mini / maxi: smallest / largest coordinate of the current
+  // positions over all axes, as computeMinMax leaves them for the query sets the reference
+  // orders).  Points of equal code keep their order (the reference's std::sort leaves it
+  // unspecified); like the reference, only the positions are reordered.
+  void order(unsigned sizeOfGrid = 64) {
+    if (positions.empty()) return;
+    Vec3f lo = positions[0], hi = positions[0];
+    for (const Vec3f& p : positions) {
+      lo.x = std::min(lo.x, p.x), lo.y = std::min(lo.y, p.y), lo.z = std::min(lo.z, p.z);
+      hi.x = std::max(hi.x, p.x), hi.y = std::max(hi.y, p.y), hi.z = std::max(hi.z, p.z);
+    }
+    const float mini = std::min(std::min(lo.x, lo.y), lo.z), maxi = std::max(std::max(hi.x, hi.y), hi.z);
+    const float norme = maxi - mini;
+    auto cell = [&](float v) {
+      const float c = std::floor((v - mini) / norme * (float)sizeOfGrid);
+      return c <= 0.f ? 0u : std::min((uint32_t)c, sizeOfGrid - 1u);
+    };
+    auto deposit = [](uint32_t v, uint32_t mask) {  // _pdep_u32
+      uint32_t r = 0;
+      for (uint32_t bit = 1; mask; bit <<= 1) {
+        const uint32_t low = mask & (~mask + 1u);
+        if (v & bit) r |= low;
+        mask &= mask - 1u;
+      }
+      return r;
+    };
+    std::vector<std::pair<uint32_t, uint32_t>> code(positions.size());
+    for (size_t i = 0; i < positions.size(); ++i)
+      code[i] = {deposit(cell(positions[i].x), 0x92492492u) | deposit(cell(positions[i].y), 0x49249249u) |
+                     deposit(cell(positions[i].z), 0x24924924u),
+                 (uint32_t)i};
+    std::stable_sort(code.begin(), code.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<Vec3f> sorted(positions.size());
+    for (size_t i = 0; i < code.size(); ++i) sorted[i] = positions[code[i].second];
+    positions.swap(sorted);
+  }
+
   // Pn::save: PN records or a binary little-endian PLY, positions mapped back to the file frame.
   void save(const std::string& path) const { save(path, positions, normals); }
   void save(const std::string& path, const std::vector<Vec3f>& pts, const std::vector<Vec3f>& nrm) const {

@@ -32,7 +32,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 __all__ = ["read_points", "write_points", "write_mesh", "mesh_comments", "read_mesh", "normalize", "Frame",
-           "read_params", "Params", "write_timings"]
+           "morton_codes", "morton_order", "read_params", "Params", "write_timings"]
 
 _PLY_TYPES = {
     "char": "i1", "int8": "i1", "uchar": "u1", "uint8": "u1", "short": "i2", "int16": "i2", "ushort": "u2",
@@ -144,6 +144,34 @@ class Frame:
     """The normalised frame: file position = p * norme + offset."""
     offset: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
     norme: float = 1.0
+
+
+def morton_codes(P: np.ndarray, size_of_grid: int = 64) -> np.ndarray:
+    """The Morton codes of Pn::order (sources/apss/Pn.cpp:388-417) for points P: per axis the cell
+    floor((p - mini) / norme * size) clamped to [0, size - 1], with mini / maxi the smallest /
+    largest coordinate of the point set's box over all axes and norme = maxi - mini (float32), bits
+    interleaved as _pdep_u32 with the masks 0x92492492 (x), 0x49249249 (y), 0x24924924 (z): bit k
+    of x at 3k + 1, of y at 3k, of z at 3k + 2."""
+    P = np.ascontiguousarray(P, np.float32)
+    lo, hi = P.min(0), P.max(0)
+    mini, maxi = np.float32(min(lo)), np.float32(max(hi))
+    norme = np.float32(maxi - mini)
+    v = np.floor(((P - mini) / norme) * np.float32(size_of_grid)).astype(np.int64)
+    v = np.clip(v, 0, size_of_grid - 1).astype(np.uint32)
+    code = np.zeros(len(P), np.uint32)
+    for k in range(11):
+        for axis, off in ((0, 1), (1, 0), (2, 2)):
+            pos = 3 * k + off
+            if pos < 32:
+                code |= ((v[:, axis] >> np.uint32(k)) & np.uint32(1)) << np.uint32(pos)
+    return code
+
+
+def morton_order(P: np.ndarray, size_of_grid: int = 64) -> np.ndarray:
+    """The permutation of Pn::order (Pn.cpp:388-445): points sorted by their Morton code
+    (morton_codes).  The reference's std::sort leaves points of equal code in an unspecified
+    order; here they keep their input order.  The reference reorders the positions only."""
+    return np.argsort(morton_codes(P, size_of_grid), kind="stable")
 
 
 def normalize(P: np.ndarray):

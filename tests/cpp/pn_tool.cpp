@@ -1,6 +1,7 @@
 // Test driver for include/mlod_pn.hpp (no GPU): pn_tool <in.ply|pn> <out_prefix> [<model.params>]
 // writes <out_prefix>.norm (float32: offset[3], norme, then the normalised positions), saves the
-// normalised set back to <out_prefix>.ply (file frame) and prints the .params fields and the
+// normalised set back to <out_prefix>.ply (file frame), the normalised positions in Pn::order's
+// Morton order to <out_prefix>.order, and prints the .params fields and the
 // kernel's sigmas for radius 0.8125; with the .params, also writes <out_prefix>Mesh.ply, a mesh
 // of the first four points (two triangles) with the mesh-file comments of that kernel.
 #include "mlod_dc.hpp"
@@ -19,6 +20,13 @@ int main(int argc, char** argv) {
   std::fwrite(pn.positions.data(), sizeof(mlod::Vec3f), pn.positions.size(), f);
   std::fclose(f);
   pn.save(out + ".ply");
+  {  // Pn::order of the normalised positions
+    mlod::Pn o = pn;
+    o.order();
+    FILE* g = std::fopen((out + ".order").c_str(), "wb");
+    std::fwrite(o.positions.data(), sizeof(mlod::Vec3f), o.positions.size(), g);
+    std::fclose(g);
+  }
   std::printf("points %zu normals %zu\n", pn.positions.size(), pn.normals.size());
   if (argc > 3) {
     mlod::ModelParams p;

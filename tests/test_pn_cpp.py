@@ -41,6 +41,8 @@ def test_cpp_matches_python(tool, tmp_path, name, ext):
     assert np.array_equal(raw[:3], fr.offset) and raw[3] == np.float32(fr.norme)
     assert np.array_equal(raw[4:].reshape(-1, 3), Pn)
     assert out[0] == f"points {len(Pn)} normals {len(Pn)}"
+    # Pn::order of the normalised positions = pointio.morton_order
+    assert np.array_equal(np.fromfile(str(tmp_path / "c.order"), np.float32).reshape(-1, 3), Pn[pointio.morton_order(Pn)])
     # the C++ save of the normalised set = the Python write of it in the file frame
     P2, N2 = pointio.read_points(str(tmp_path / "c.ply"))
     pointio.write_points(str(tmp_path / "py.ply"), Pn, g["normals"], fr)

@@ -146,3 +146,33 @@ def test_mesh_file_round_trip(tmp_path):
     assert np.array_equal(N2, N) and np.array_equal(F2, F)
     head = open(path, "rb").read().split(b"end_header\n")[0].split(b"\n")
     assert b"element face 30" in head and b"property list uchar int vertex_indices" in head
+
+
+def _pdep(v, mask):
+    """_pdep_u32 by definition: the low bits of v deposited, in order, at the set bits of mask."""
+    r, k = 0, 0
+    for pos in range(32):
+        if mask >> pos & 1:
+            if v >> k & 1:
+                r |= 1 << pos
+            k += 1
+    return r
+
+
+def test_morton_order():
+    """Pn::order (Pn.cpp:388-445): codes against _pdep_u32 written from its definition, cells from
+    the box over all axes (the largest coordinate lands in the last cell), a stable sort."""
+    rng = np.random.default_rng(5)
+    P = (rng.standard_normal((3000, 3)) * [1.0, 0.5, 0.25]).astype(np.float32)
+    P[:50] = P[50:100]  # equal codes
+    code = pointio.morton_codes(P)
+    mini, maxi = np.float32(P.min()), np.float32(P.max())
+    for i in range(0, len(P), 37):
+        c = [min(max(int(np.floor((P[i, a] - mini) / (maxi - mini) * np.float32(64))), 0), 63) for a in range(3)]
+        assert code[i] == _pdep(c[0], 0x92492492) | _pdep(c[1], 0x49249249) | _pdep(c[2], 0x24924924)
+    j = np.unravel_index(np.argmax(P), P.shape)
+    assert ((code[j[0]] >> np.array([1, 0, 2])[j[1]]) & 1) == 1  # the maximum's axis is in the top cell
+    perm = pointio.morton_order(P)
+    assert np.all(np.diff(code[perm].astype(np.int64)) >= 0)
+    eq = code[perm][1:] == code[perm][:-1]
+    assert np.all(perm[1:][eq] > perm[:-1][eq])  # equal codes keep their input order
