@@ -61,6 +61,20 @@ def test_argument_errors_without_gpu(L):
     assert L.mlod_tree_destroy(None) == 0
     info = _lib.TreeInfoC()
     assert L.mlod_tree_get_info(None, C.byref(info)) == -1
+    # dual contouring and mesh passes: NULL handles and arrays are rejected before any HIP call
+    n = C.c_uint32()
+    assert L.mlod_dc_build(None, None, 0, 0, 0, None) == -1
+    assert L.mlod_dc_build_from_points(None, None, 0, None) == -1
+    assert L.mlod_dc_extract(None, C.byref(n), C.byref(n), None) == -1
+    assert L.mlod_dc_refine(None, None, 3, 2, C.byref(n), C.byref(n), None) == -1
+    assert L.mlod_dc_refined_mesh(None, None, None, None, None) == -1
+    assert L.mlod_mesh_smooth(None, 4, None, 0, None, None) == -1
+    tri = np.array([[0, 1, 9]], np.uint32)
+    assert L.mlod_mesh_smooth(fp, 4, tri.ctypes.data_as(C.c_void_p), 1, fp, None) == -1
+    assert b"not a vertex" in L.mlod_last_error()
+    assert L.mlod_mesh_subdivide(fp, 4, tri.ctypes.data_as(C.c_void_p), 1, fp, 4, tri.ctypes.data_as(C.c_void_p),
+                                 C.byref(n), None) == -1
+    assert b"fewer than" in L.mlod_last_error()
 
 
 def test_kernel_sigma_generation_matches_reference_semantics():
