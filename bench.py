@@ -484,7 +484,7 @@ def main():
         except Exception as ex:  # the baseline is reported, never the measured product
             log(f"cpu baseline failed: {ex}")
     out = {
-        "metric": "projected pts/s (8M-pt MLoD, <=1e-6*diag vs FP64) vs plain; serve p50/p99 ms",
+        "metric": "projected pts/s (8M-pt MLoD, \u22641e-6\u00b7diag vs FP64) vs plain; serve p50/p99 ms",
         "value": head["pts_per_s"],
         "unit": "projected pts/s",
         "n_gpus": 1,
