@@ -61,3 +61,28 @@ def test_full_size_projection_gate_and_batch_invariance(workload):
     idx = np.arange(0, len(Q), 997)
     sp, sn = ap.project_points(Q[idx], 3)
     assert np.array_equal(sp, out["fp32c"][0][idx]) and np.array_equal(sn, out["fp32c"][1][idx])
+
+
+def test_full_size_surface_mesh_is_closed(workload):
+    """The surface mesh of the full cloud at depth 8 (the bench's mesh): the superquadric is closed,
+    so dual contouring of its projected corner field gives a closed, consistently oriented mesh of
+    genus 0 (every directed edge once, its reverse once, V - E + F = 2), and the refined vertices lie
+    on the cloud."""
+    from mlodsurfaces_amd import dc
+    p = ds.pn_normalize(workload["points"])
+    t = A.APSSOctree().build(p, workload["normals"])
+    ap = A.Fastapss(t, A.Kernel(**workload["kernel"]))
+    ap.setScalingProtectionSphere(workload["cutoff"])
+    ap.set_precision("fp32c")
+    V, N, F, info = dc.surface_mesh(ap, 8, refine=0)
+    assert len(F) > 100000
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]]).astype(np.int64)
+    key = e[:, 0] * len(V) + e[:, 1]
+    rev = e[:, 1] * len(V) + e[:, 0]
+    assert len(np.unique(key)) == len(key)  # no directed edge twice
+    assert np.isin(rev, key).all()  # every edge has its opposite
+    assert len(V) - len(key) // 2 + len(F) == 2
+    from scipy.spatial import cKDTree
+    d, _ = cKDTree(p[::4]).query(V)
+    print(f"depth-8 mesh: {len(V)} vertices, {len(F)} triangles, vertex-to-cloud median {np.median(d):.2e}")
+    assert np.median(d) < 2e-3
