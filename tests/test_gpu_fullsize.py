@@ -86,3 +86,34 @@ def test_full_size_surface_mesh_is_closed(workload):
     d, _ = cKDTree(p[::4]).query(V)
     print(f"depth-8 mesh: {len(V)} vertices, {len(F)} triangles, vertex-to-cloud median {np.median(d):.2e}")
     assert np.median(d) < 2e-3
+
+
+@pytest.mark.parametrize("name", ["sphere100k", "torus1m"])
+def test_full_size_small_configs(oracle_built, name):
+    """The 100k-point sphere (multi-Gaussian S = 4, cutoff 3) and the 1M-point torus (rational kernel
+    e = 4, cutoff 3) at full size with the reference CPU function (REF_CPU): the compensated form
+    within 1e-6 x diag of the FP64 form on every query after 3 steps, and of the FP64 oracle (same
+    tie policy) on a sample of 2000 queries."""
+    w = ds.make_workload(name)
+    P, N, Q = w["points"], w["normals"], w["queries"]
+    t = A.APSSOctree().build(P, N)
+    K = A.Kernel(**w["kernel"])
+    diag = float(np.linalg.norm(P.max(0).astype(np.float64) - P.min(0)))
+    out = {}
+    for prec in ("fp32c", "fp64"):
+        ap = A.Fastapss(t, K)
+        ap.setScalingProtectionSphere(w["cutoff"])
+        ap.set_semantics("ref_cpu")
+        ap.set_precision(prec)
+        ap.copyPointsToGPU(len(Q), Q)
+        out[prec] = ap.project(len(Q), np.zeros_like(Q), np.zeros_like(Q), 3, K)
+    err = np.linalg.norm(out["fp32c"][0].astype(np.float64) - out["fp64"][0], axis=1) / diag
+    idx = np.random.default_rng(7).choice(len(Q), 2000, replace=False)
+    o = oracle_built.OracleTree(P, N)
+    k = oracle_built.make_kernel(type=K.type, sigma=K.sigma, exponent=K.exponent, constant_shift=K.constant_shift,
+                                 sigmas=K.sigmas)
+    par = oracle_built.make_params(cutoff=w["cutoff"], min_depth=0, semantics="ref_cpu")
+    op, _ = o.project(Q[idx], k, par, n_iter=3, precision=1, nthreads=16)
+    oerr = np.linalg.norm(out["fp32c"][0][idx].astype(np.float64) - op, axis=1) / diag
+    print(f"{name}: FP32C vs FP64 max {err.max():.3e}, vs the FP64 oracle (2000 queries) max {oerr.max():.3e} x diag")
+    assert err.max() < 1e-6 and oerr.max() < 1e-6
