@@ -117,3 +117,65 @@ def test_full_size_small_configs(oracle_built, name):
     oerr = np.linalg.norm(out["fp32c"][0][idx].astype(np.float64) - op, axis=1) / diag
     print(f"{name}: FP32C vs FP64 max {err.max():.3e}, vs the FP64 oracle (2000 queries) max {oerr.max():.3e} x diag")
     assert err.max() < 1e-6 and oerr.max() < 1e-6
+
+
+@pytest.mark.timeout(400)
+def test_full_size_c5_neighbourhoods(oracle_built):
+    """kNN (k = 32) and ball (r = 3 sigma_0) APSS at the C5 size (20M points, queries from the
+    dual-contouring grid): for a sample of queries, one FP64 step on the GPU equals the reference's
+    fit (the oracle's FP64 projectOnAPSSStats) over the neighbourhood an exact k-d tree finds, with
+    unit-area weights of the reference's multi-Gaussian kernel at the raw points
+    (CPUUtils.cpp:690-737).  The library returns float32 positions, so the two agree to that
+    rounding (~1e-8 x diag); one neighbour more or less moves a fit by ~1e-5 x diag."""
+    from scipy.spatial import cKDTree
+    from mlodsurfaces_amd import dc
+    w = ds.make_workload("c5_knn")
+    P, N = w["points"], w["normals"]
+    t = A.APSSOctree().build(P, N)
+    K = A.Kernel(**w["kernel"])
+    ap = A.Fastapss(t, K)
+    ap.setScalingProtectionSphere(w["cutoff"])
+    Q = dc.grid_centres(ap, 11)
+    q = Q[np.random.default_rng(11).choice(len(Q), 150, replace=False)]
+    P64, N64 = P.astype(np.float64), N.astype(np.float64)
+    kd = cKDTree(P64)
+    diag = float(np.linalg.norm(P64.max(0) - P64.min(0)))
+    sig = np.asarray(K.sigmas, np.float32).astype(np.float64)  # the kernel as installed (float sigmas)
+    r = 3.0 * sig[0]
+    R = float(t.radius())
+
+    def fit(qi, j):
+        # sums in the query's frame (the neighbourhood spans ~1e-3 of the cloud, so global-frame
+        # sums would lose five digits to cancellation before the fit)
+        p, n = P64[j] - qi, N64[j]
+        d2 = (p * p).sum(1)
+        wgt = (np.exp(-d2[:, None] / (2 * sig[None] ** 2)) / sig[None] ** 3).sum(1)
+        s = np.concatenate([[wgt.sum()], wgt @ p, wgt @ n, [wgt @ (p * n).sum(1)], [wgt @ d2]])
+        return oracle_built.oracle_fit(s, np.zeros(3), R, ref_gpu=False)[0] + qi
+
+    for mode in ("knn", "ball"):
+        if mode == "knn":
+            t.knnMode(True, 32)
+        else:
+            t.ballMode(True, -1.0)
+        g = A.Fastapss(t, K)
+        g.setScalingProtectionSphere(w["cutoff"])
+        g.set_semantics("ref_cpu")
+        g.set_precision("fp64")
+        gp, _ = g.project_points(q, 1)
+        errs = []
+        for i, qi in enumerate(q.astype(np.float64)):
+            if mode == "knn":
+                dist, j = kd.query(qi, 33)
+                if dist[32] - dist[31] < 1e-9:  # a tie at the k-th neighbour: either set is right
+                    continue
+                j = j[:32]
+            else:
+                j = np.asarray(kd.query_ball_point(qi, r), np.int64)
+                if len(j) < 10:  # a fit on a few points is decided by rounding
+                    continue
+            errs.append(np.linalg.norm(gp[i] - fit(qi, j)) / diag)
+        errs = np.array(errs)
+        print(f"C5 {mode}: {len(errs)} queries, GPU FP64 vs the exact neighbourhood's fit: median {np.median(errs):.1e}, "
+              f"max {errs.max():.1e} x diag")
+        assert len(errs) > 100 and errs.max() < 1e-7
