@@ -149,7 +149,44 @@ def mesh_measure(ap, depth, label):
     return out
 
 
-def workload_mesh_measure(A, ds, w, depth, semantics, precision):
+def dc_cpu_baseline(A, ap, depth, gpu):
+    """CPU baseline of the dual-contouring stages (kind "port": the oracle's sequential restatement
+    of the reference's uniformize, minimal-edge extraction and mesh passes, one thread), on the same
+    crossed cells and the GPU's corner values; the reference's own code needs Eigen and is not built."""
+    from mlodsurfaces_amd.dc import DualContouring
+    from oracle import bind
+    total = []
+    gen = A.OctreeGenerator(ap, None)
+    gen.generate(depth, total)
+    gen.close()
+    cells = total[0]
+    g = DualContouring(cells, depth)
+    ap._sync_params()
+    g.project_corners(ap.m_apssOctree)
+    xyz, _ = g.corners()
+    f = g.corner_values()
+    g.close()
+    t0 = time.perf_counter()
+    o = bind.OracleDC(cells, depth)
+    t1 = time.perf_counter()
+    o.set_corners(xyz, f)
+    v, vn, tri = o.extract()
+    t2 = time.perf_counter()
+    for _ in range(2):
+        v = bind.oracle_mesh_smooth(v, tri)
+        v, tri = bind.oracle_mesh_subdivide(v, tri)
+    t3 = time.perf_counter()
+    out = {"kind": "port", "cores": 1, "uniformize_s": t1 - t0, "extract_s": t2 - t1, "mesh_passes_s": t3 - t2,
+           "triangles": int(len(tri)),
+           "sample": f"the oracle's uniformize, extraction and two smoothing + subdivision rounds (no projection) "
+                     f"on the same {len(cells)} cells and corner values, single-threaded"}
+    log(f"surface mesh CPU port: uniformize {out['uniformize_s']:.2f} s (GPU {gpu['ms_uniformize']:.1f} ms), "
+        f"extract {out['extract_s']:.2f} s (GPU {gpu['ms_extract']:.1f} ms), smoothing + subdivision "
+        f"{out['mesh_passes_s']:.2f} s")
+    return out
+
+
+def workload_mesh_measure(A, ds, w, depth, semantics, precision, cpu=True):
     """The surface mesh of the workload's cloud in the reference's normalised frame."""
     p = ds.pn_normalize(w["points"])
     t = A.APSSOctree().build(p, w["normals"])
@@ -158,7 +195,13 @@ def workload_mesh_measure(A, ds, w, depth, semantics, precision):
     ap.setScalingProtectionSphere(w["cutoff"])
     ap.set_semantics(semantics)
     ap.set_precision(precision)
-    return mesh_measure(ap, depth, "workload")
+    out = mesh_measure(ap, depth, "workload")
+    if cpu:
+        try:
+            out["cpu_baseline"] = dc_cpu_baseline(A, ap, depth, out)
+        except Exception as ex:  # the baseline is reported, never the measured product
+            log(f"surface mesh cpu baseline failed: {ex}")
+    return out
 
 
 def real_models_measure(A):
@@ -464,7 +507,8 @@ def main():
     hiprt.device_synchronize()
     mesh = None
     if args.mesh_depth > 0 and args.workload == "superquadric8m" and args.scale == 1.0:
-        mesh = workload_mesh_measure(A, ds, w, args.mesh_depth, args.semantics, args.precision)
+        mesh = workload_mesh_measure(A, ds, w, args.mesh_depth, args.semantics, args.precision,
+                                     cpu=not args.no_cpu_baseline)
     real = None
     if args.workload == "superquadric8m" and args.scale == 1.0 and not args.no_real_models:
         real = real_models_measure(A)
