@@ -360,7 +360,7 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only.  The projection path would shard by "
+        print("bench.py: measures one GPU only.  The projection path would shard by "
               "replicating the octree and splitting the Morton-ordered queries across GPUs; that is not "
               "built here.  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
