@@ -211,7 +211,7 @@ def real_models_measure(A):
     to the reference's projectCPU results stored with the model (4096 queries, 3 steps)."""
     from mlodsurfaces_amd import hiprt, pointio
     out = {}
-    for name in ("Face", "half-igea"):
+    for name in ("Face", "half-igea", "gargoyle_0"):
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", f"real_{name}.npz")
         if not os.path.exists(path):
             continue

@@ -1,14 +1,14 @@
 """Writes tests/golden/real_<model>.npz from the reference's own data and CPU path.
 
-For two of the reference's sample models (data/Face.ply with Face.params: 8 sigmas, cutoff 1.4,
-min depth 1; data/half-igea.ply with half-igea.params: 15 sigmas, cutoff 2, kept to a seeded 24k-
-point subset) it stores the raw points and normals as read from the PLY file, the 134-byte
+For three of the reference's sample models (data/Face.ply with Face.params: 8 sigmas, cutoff 1.4,
+min depth 1; data/half-igea.ply with half-igea.params: 15 sigmas, cutoff 2; data/gargoyle_0.ply with
+gargoyle_0.params: 10 sigmas, cutoff 1.4; the last two kept to seeded 24k-point subsets) it stores the raw points and normals as read from the PLY file, the 134-byte
 .params file, seeded queries (input points jittered in the normalised frame) and the results of
 the reference's APSSOctree::projectCPU (REF_CPU; oracle/_ref/libmlodref.so, compiled from
 sources/apss/CPUUtils.cpp and Kernel.cpp where they lie) on the cloud normalised as
 Pn::normalize does, with the kernel of the .params file (Kernel::updateSigmas), after 1 and 3
 steps, plus the reference octree's node count.  No reference code is stored.  Re-run with
-    make -C oracle all ref && python tests/golden/make_realdata.py
+    make -C oracle all ref && python tests/golden/make_realdata.py [model ...]
 """
 from __future__ import annotations
 
@@ -26,11 +26,13 @@ from oracle import bind  # noqa: E402
 
 DATA = "/root/reference/data"
 SEED = 12345
-MODELS = {"Face": None, "half-igea": 24000}
+MODELS = {"Face": None, "half-igea": 24000, "gargoyle_0": 24000}
 
 
-def main():
+def main(names=None):
     for name, keep in MODELS.items():
+        if names and name not in names:
+            continue
         P, N = pointio.read_points(os.path.join(DATA, name + ".ply"))
         if keep is not None and keep < len(P):
             idx = np.sort(np.random.default_rng(7).choice(len(P), keep, replace=False))
@@ -68,4 +70,4 @@ def ref_radius(ref):
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
