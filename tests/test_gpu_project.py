@@ -58,6 +58,8 @@ def test_matches_reference_cpu_golden(oracle_built, name, precision):
 KERNELS = {
     "mg4": lambda R: dict(type=2, sigmas=[0.01 * 2.0 ** i * R for i in range(4)]),
     "mg6": lambda R: dict(type=2, sigmas=[0.01 * 2.0 ** i * R for i in range(6)]),
+    # the most sigmas a kernel holds (the sample models' .params files go up to 30)
+    "mg32": lambda R: dict(type=2, sigmas=[0.004 * 1.2 ** i * R for i in range(32)]),
     "rational": lambda R: dict(type=1, exponent=4.0, constant_shift=1e-3),
     "rational_e2.5": lambda R: dict(type=1, exponent=2.5, constant_shift=2e-3),
     "gauss": lambda R: dict(type=0, sigma=0.03 * R, constant_shift=0.0),
