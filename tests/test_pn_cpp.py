@@ -112,6 +112,14 @@ def test_cpp_surface_mesh_matches_python(tmp_path):
     a = open(str(tmp_path / "cpp.ply"), "rb").read()
     b = open(str(tmp_path / "FaceMesh.ply"), "rb").read()
     assert len(a) > 100000 and a == b
+    # the octree built on the points (computeDualContouringOnPoints, uniformizeFromPoints)
+    r = subprocess.run([exe, model, str(tmp_path / "Face.params"), str(tmp_path / "cpp_pts.ply"), "points"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    pointio.project_model(model, log=lambda m: None, mesh_depth=0, mesh_octree="points")
+    a = open(str(tmp_path / "cpp_pts.ply"), "rb").read()
+    b = open(str(tmp_path / "FaceMesh.ply"), "rb").read()
+    assert len(a) > 100000 and a == b
 
 
 @pytest.mark.gpu
