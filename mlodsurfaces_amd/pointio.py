@@ -32,7 +32,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 __all__ = ["read_points", "write_points", "write_mesh", "mesh_comments", "read_mesh", "normalize", "Frame",
-           "morton_codes", "morton_order", "read_params", "Params", "write_timings"]
+           "morton_codes", "morton_order", "read_params", "Params", "write_timings", "compare_point_set"]
 
 _PLY_TYPES = {
     "char": "i1", "int8": "i1", "uchar": "u1", "uint8": "u1", "short": "i2", "int16": "i2", "ushort": "u2",
@@ -368,6 +368,22 @@ def write_timings(path: str, model: str, n_input: int, n_projected: int, step_ms
              "Crossed nodes: 0", "Nodes created on projected points: 0"]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def compare_point_set(apss, points, frame: Frame | None = None, n_iterations: int = 1):
+    """scene::comparePointSet (scene.cpp:1663-1720): a second point set (a path, or (n, 3) points
+    in the file frame) mapped into the model's normalised frame, projected `n_iterations` steps
+    with `apss`'s tree and settings, and its signed distances (p - p')·n' to the surface, also
+    divided by their largest magnitude (the values the reference colours with, in [-1, 1]).
+    The reference's GPU plugin projects its uploaded queries there rather than the loaded set;
+    this projects the loaded set.  Returns (values, distances), distances in the normalised frame."""
+    P = read_points(points)[0] if isinstance(points, str) else np.asarray(points, np.float32)
+    if frame is not None:
+        P = ((P - np.asarray(frame.offset, np.float32)) / np.float32(frame.norme)).astype(np.float32)
+    op, on = apss.project_points(P, n_iterations)
+    d = np.einsum("ij,ij->i", (P - op).astype(np.float64), on.astype(np.float64)).astype(np.float32)
+    norme = max(abs(float(d.max())), abs(float(d.min()))) if len(d) else 0.0
+    return (d / np.float32(norme) if norme > 0 else d), d
 
 
 def project_model(model: str, params_path: str | None = None, out: str | None = None, precision="fp32c",

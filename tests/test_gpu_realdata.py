@@ -116,3 +116,26 @@ def test_project_model_mesh_on_points(tmp_path):
     cover, _ = cKDTree(V).query(g["points"])
     print(f"points-octree mesh {len(V)} vertices; sample-to-mesh / gap: median {np.median(cover) / gap:.2f}")
     assert np.median(cover) < 2 * gap
+
+
+def test_compare_point_set():
+    """scene::comparePointSet: points off a sphere by known radial amounts get those signed
+    distances to the projected surface, and the colour values are them over their largest."""
+    from mlodsurfaces_amd import apss as A
+    from mlodsurfaces_amd import datasets as ds
+    p, n = ds.sphere(40000, seed=12, noise=0.0)
+    p = (p * 0.6).astype(np.float32)
+    t = A.APSSOctree().build(p, n)
+    k = A.Kernel(type=2, sigmas=[float(np.float32(0.01 * 2 ** i) * np.float32(t.radius())) for i in range(4)])
+    ap = A.Fastapss(t, k)
+    ap.setScalingProtectionSphere(2.0)
+    ap.set_precision("fp32c")
+    u, _ = ds.sphere(500, seed=13, noise=0.0)
+    delta = np.random.default_rng(14).uniform(-0.01, 0.01, len(u))
+    frame = pointio.Frame(np.array([1.0, 2.0, -3.0], np.float32), 2.0)
+    q = (u * (0.6 + delta)[:, None]).astype(np.float32)
+    vals, d = pointio.compare_point_set(ap, q * np.float32(2.0) + frame.offset, frame, n_iterations=3)
+    qn = ((q * np.float32(2.0) + frame.offset - frame.offset) / np.float32(2.0)).astype(np.float32)
+    expect = np.linalg.norm(qn.astype(np.float64), axis=1) - 0.6
+    assert np.abs(d - expect).max() < 1e-4
+    assert abs(np.abs(vals).max() - 1.0) < 1e-6 and np.allclose(vals * np.abs(d).max(), d, atol=1e-7)
