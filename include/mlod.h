@@ -163,29 +163,45 @@ int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out6, void* stream);
  * device and, when it completes, takes the most urgent resident or queued work for the next one.
  * Queries stay in a device pool between steps; bulk launches take two steps, interactive ones
  * up to 16 at raised wavefront priority; a request's results reach its buffers as soon as its
- * own workgroups finish.  Each query gives the same result alone, in any mix and through
- * mlod_project.  Registered trees must outlive the server;
- * request buffers must stay valid until the request's completion is polled. */
+ * own workgroups finish.  Chunks are admitted to the pool by memory footprint (36 B per query
+ * slot, 68 B in the FP64 form) and wait in their queue while it is full.  Each query gives the
+ * same result alone, in any mix and through mlod_project.  Registered trees must outlive the
+ * server; request buffers must stay valid until the request's completion is polled. */
 typedef struct mlod_server mlod_server;
 typedef struct {
   uint64_t ticket;
   double latency_ms;   /* submit -> all results copied to the caller's buffers */
   double queue_ms;     /* submit -> first step launch of the request */
   double completed_s;  /* completion time since server creation */
+  int32_t status;      /* 0 completed, 1 cancelled (mlod_server_cancel) */
+  int32_t reserved;
 } mlod_server_done;
 
-/* chunk_queries = 0: max_batch_queries / 4 (rounded to 128) */
+/* chunk_queries = 0: max_batch_queries / 4 (rounded to 128); precision: any MLOD_PREC_* form */
 int mlod_server_create(uint32_t max_batch_queries, uint32_t chunk_queries, uint32_t inflight, int32_t semantics,
                        int32_t precision, mlod_server** out);
 int mlod_server_destroy(mlod_server* s);
+/* Registers a resident tree; MLOD_ERR_OOM when it would exceed the server's HBM budget. */
 int mlod_server_add_tree(mlod_server* s, mlod_tree* t, uint32_t* tree_id);
-/* n_iter projection steps of m queries (host, m x 3, any m >= 1) on tree tree_id with kernel k;
- * results go to out_pts / out_nmls (host, m x 3, either may be NULL).  step_max < 0 => the tree's
- * radius.  Multi-Gaussian kernels take at most 8 sigmas here.  n_iter = 0 completes at once with
- * the queries unchanged and zero normals. */
+/* HBM budget of the server (0 = none): the registered trees, the staging buffers and the query
+ * pool together hold at most hbm_bytes of device memory.  The pool is reallocated to what the
+ * budget leaves (at most its default size); MLOD_ERR_OOM when that is under two chunks, or when
+ * the trees already exceed it.  Only while no request is outstanding (else MLOD_ERR_STATE). */
+int mlod_server_set_budget(mlod_server* s, uint64_t hbm_bytes);
+/* [budget, tree bytes, staging bytes, pool bytes, pool query slots] */
+int mlod_server_memory(mlod_server* s, uint64_t* out5);
+/* n_iter projection steps of m queries (host, m x 3, any m >= 1) on tree tree_id with kernel k
+ * (checked as by mlod_set_kernel, up to 32 sigmas); results go to out_pts / out_nmls (host, m x 3,
+ * either may be NULL).  cutoff >= 0; step_max < 0 => the tree's radius.  n_iter = 0 completes at
+ * once with the queries unchanged and zero normals.  MLOD_ERR_OOM when a chunk of the request can
+ * never be resident in the (budgeted) pool. */
 int mlod_server_submit(mlod_server* s, uint32_t tree_id, const mlod_kernel* k, float cutoff, uint32_t min_depth,
                        uint32_t n_iter, float step_max, const float* queries, uint32_t m, float* out_pts,
                        float* out_nmls, uint64_t* ticket);
+/* Cancels an outstanding request: its queued chunks and its runs waiting in the pool are dropped
+ * at the next launch boundary, runs in a launch are discarded when it completes; the request then
+ * completes with status 1 (its buffers hold the runs handed back before, if any). */
+int mlod_server_cancel(mlod_server* s, uint64_t ticket);
 /* Completed requests since the last poll (waits up to timeout_s seconds for one). */
 int mlod_server_poll(mlod_server* s, mlod_server_done* out, uint32_t cap, uint32_t* n_out, double timeout_s);
 /* step launches, requests completed, queries completed, chunks queued */

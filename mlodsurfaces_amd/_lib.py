@@ -46,7 +46,7 @@ class TreeInfoC(C.Structure):
 
 class ServerDoneC(C.Structure):
     _fields_ = [("ticket", C.c_uint64), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
-                ("completed_s", C.c_double)]
+                ("completed_s", C.c_double), ("status", C.c_int32), ("reserved", C.c_int32)]
 
 
 SIGNATURES = {
@@ -72,6 +72,9 @@ SIGNATURES = {
                                      _vp, C.c_uint32, _vp, _vp, _u64]),
     "mlod_server_poll": (C.c_int, [_vp, C.POINTER(ServerDoneC), C.c_uint32, _u, C.c_double]),
     "mlod_server_stats": (C.c_int, [_vp, _u64]),
+    "mlod_server_set_budget": (C.c_int, [_vp, C.c_uint64]),
+    "mlod_server_memory": (C.c_int, [_vp, _u64]),
+    "mlod_server_cancel": (C.c_int, [_vp, C.c_uint64]),
     "mlod_project_points": (C.c_int, [_vp, _f, C.c_uint32, C.c_uint32, C.c_float, _f, _f, _vp]),
     "mlod_queries_reorganize": (C.c_int, [_vp, _u, C.c_uint32, C.c_uint32, _vp]),
     "mlod_queries_fit_kinds": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),

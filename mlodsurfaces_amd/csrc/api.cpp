@@ -45,15 +45,19 @@ static float bbox_radius(const float bb[3], const float BB[3]) {
   return std::sqrt((float)d2) / 2.f;
 }
 
-static int validate_kernel(const mlod_kernel* k) {
+int validate_kernel(const mlod_kernel* k) {
   if (!k) return fail(MLOD_ERR_INVALID, "kernel is NULL");
   if (k->type < 0 || k->type > 2) return fail(MLOD_ERR_INVALID, "kernel type must be 0, 1 or 2 (Kernel.h:37)");
   if (k->type == MLOD_KERNEL_GAUSSIAN_MULTIPLE && (k->n_sigmas < 1 || k->n_sigmas > MLOD_MAX_SIGMAS))
     return fail(MLOD_ERR_INVALID, "multi-Gaussian kernel needs 1..32 sigmas");
   if (k->type == MLOD_KERNEL_GAUSSIAN_MULTIPLE)
     for (int s = 0; s < k->n_sigmas; ++s)
-      if (!(k->sigmas[s] > 0.f)) return fail(MLOD_ERR_INVALID, "sigmas must be positive");
-  if (k->type == MLOD_KERNEL_GAUSSIAN && !(k->sigma > 0.f)) return fail(MLOD_ERR_INVALID, "sigma must be positive");
+      if (!(k->sigmas[s] > 0.f) || !std::isfinite(k->sigmas[s]))
+        return fail(MLOD_ERR_INVALID, "sigmas must be positive and finite");
+  if (k->type == MLOD_KERNEL_GAUSSIAN && (!(k->sigma > 0.f) || !std::isfinite(k->sigma)))
+    return fail(MLOD_ERR_INVALID, "sigma must be positive and finite");
+  if (k->type == MLOD_KERNEL_SINGULAR && !std::isfinite(k->exponent))
+    return fail(MLOD_ERR_INVALID, "exponent must be finite");
   return MLOD_OK;
 }
 

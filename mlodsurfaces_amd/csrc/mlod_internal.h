@@ -64,25 +64,11 @@ struct alignas(16) KParams {
   double bd[MLOD_MAX_SIGMAS];  // FP64: 1 / s^3
 };
 
-// Kernel coefficients of one serving request (same fields and meaning as KParams, at most 8
-// sigmas), read per lane from the request table.
-constexpr int kServeMaxSigmas = 8;
-struct WCoef {
-  int32_t type;
-  int32_t ns;
-  int32_t eint;
-  float c;
-  float cp;
-  float e;
-  float a[kServeMaxSigmas];
-  float b[kServeMaxSigmas];
-};
-
 // One run of a serving step launch (a resident range of one request's queries): its tree,
 // kernel, parameters and place in the server's query pool.
 struct ReqDesc {
   TreeView tv;
-  WCoef w;
+  KParams kp;       // read by reference from the launch table (wave-uniform scalar loads)
   float cutoff;
   float stepMax;
   uint32_t minDepth;
@@ -161,6 +147,8 @@ namespace mlod {
 
 // error plumbing (api.cpp)
 int fail(int code, const std::string& msg);
+// argument checks of a weight kernel (Kernel.h:37-46): type, sigma count, positive sigmas (api.cpp)
+int validate_kernel(const mlod_kernel* k);
 std::string g_last_error();  // the calling thread's last message
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
@@ -199,9 +187,10 @@ int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream
 constexpr uint32_t kBatchWG = 128;
 // One launch over nblk pool blocks: block b holds queries [d_blocks[b].y * kBatchWG, +kBatchWG) of run
 // d_reqs[d_blocks[b].x] (those below its pad0 + count); prio != 0 raises the wavefronts' priority.
+// d_qd: FP64 positions of the pool slots (the FP64 form carries them between launches), else NULL.
 int launch_project_batch(const ReqDesc* d_reqs, const uint2* d_blocks, uint32_t nblk, float4* d_q, float4* d_qn,
-                         const BatchOut& out, uint32_t prio, int semantics, int precision, hipStream_t s);
-WCoef make_wcoef(const mlod_kernel& k);
+                         double4* d_qd, const BatchOut& out, uint32_t prio, int semantics, int precision,
+                         hipStream_t s);
 
 // neighbors.hip
 int launch_neighbors(mlod_tree* t, uint32_t n, uint32_t n_iter, hipStream_t s);

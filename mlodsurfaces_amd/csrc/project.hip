@@ -58,8 +58,7 @@ __device__ __forceinline__ float weight32(const KP& k, float d2) {
   } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
     // sum over s of sigma_s^-3 * 2^(a_s d2), in order of s
     float w = k.b[0] * __builtin_amdgcn_exp2f(d2 * k.a[0]);
-    constexpr int kCap = sizeof(KP::a) / sizeof(float);  // 8 for the serving coefficients
-    if (kCap <= 8 || k.ns <= 8) {
+    if (k.ns <= 8) {
       // statically indexed, so the coefficients sit in scalar registers; the wave-uniform exit
       // is one scalar compare and branch per term (the empty asm keeps the compiler from
       // hoisting the compares into lane masks), so no term beyond ns is evaluated
@@ -70,7 +69,7 @@ __device__ __forceinline__ float weight32(const KP& k, float d2) {
         if (s >= ns) break;
         w = fmaf(k.b[s], __builtin_amdgcn_exp2f(d2 * k.a[s]), w);
       }
-    } else if constexpr (kCap > 8) {
+    } else {
       // more sigmas (the sample models' settings use up to 30): the first eight from scalar
       // registers, the rest four at a time, each group's coefficients fetched by two 16-byte
       // scalar loads (the table is 16-byte aligned) rather than one dependent load per term
@@ -111,7 +110,11 @@ __device__ __forceinline__ float weight32(const KP& k, float d2) {
 
 template <int KT>
 __device__ __forceinline__ double weight64(const KParams& k, double d2) {
-  if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
+  if constexpr (KT < 0) {
+    if (k.type == MLOD_KERNEL_GAUSSIAN_MULTIPLE) return weight64<MLOD_KERNEL_GAUSSIAN_MULTIPLE>(k, d2);
+    if (k.type == MLOD_KERNEL_GAUSSIAN) return weight64<MLOD_KERNEL_GAUSSIAN>(k, d2);
+    return weight64<MLOD_KERNEL_SINGULAR>(k, d2);
+  } else if constexpr (KT == MLOD_KERNEL_GAUSSIAN_MULTIPLE) {
     double w = 0.0;
     for (int s = 0; s < k.ns; ++s) w += exp(d2 * k.ad[s]) * k.bd[s];
     return w == 0.0 ? (double)k.c : w;
@@ -704,16 +707,18 @@ __global__ __launch_bounds__(PBS) void k_global(TreeView tv, KParams kp, float4*
 }
 
 // Serving step launch: workgroup b projects the kBatchWG pool slots of pool block blocks[b].y
-// with the tree, kernel and parameters of run descriptor blocks[b].x, read once per workgroup
-// into scalar registers; per-lane work and results are those of k_project, so a query gives the
-// same bits alone, in any mix, and whether its steps run in one launch or several.  A launch for
-// interactive requests raises its wavefronts' issue priority (s_setprio), so they keep close to
-// their unloaded speed beside bulk launches.  A run taking its last steps writes its results to
-// the host staging and flags each of its workgroups done (system-scope release).
+// with the tree, kernel and parameters of run descriptor blocks[b].x, read by the workgroup from
+// the launch table with wave-uniform (scalar) loads; per-lane work and results are those of
+// k_project, so a query gives the same bits alone, in any mix, and whether its steps run in one
+// launch or several (the FP64 form carries its FP64 position between launches in qdp, as
+// k_project carries it between the steps of one launch).  A launch for interactive requests
+// raises its wavefronts' issue priority (s_setprio), so they keep close to their unloaded speed
+// beside bulk launches.  A run taking its last steps writes its results to the host staging and
+// flags each of its workgroups done (system-scope release).
 template <int FORM, int SEM, int KT>
 __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict__ reqs, const uint2* __restrict__ blocks,
-                                                       float4* __restrict__ q, float4* __restrict__ qn, BatchOut bo,
-                                                       uint32_t prio) {
+                                                       float4* __restrict__ q, float4* __restrict__ qn,
+                                                       double4* __restrict__ qdp, BatchOut bo, uint32_t prio) {
   static_assert(PBS == kBatchWG, "one workgroup per pool block");
   using T = typename FormT<FORM>::T;
   __shared__ uint32_t sNode[kLevels][PBS];
@@ -728,25 +733,36 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
   const uint32_t i = pb * PBS + tid, outOff = R.outOff;
   if (i < R.pad0 + R.count) {
     const TreeView tv = R.tv;
-    const WCoef w = R.w;
+    const KParams& kp = R.kp;
     const float cutoff = R.cutoff, stepMax = R.stepMax;
     const uint32_t minDepth = R.minDepth, nIter = R.nIter;
     const LevelState<T> st{sNode, sMask, sMul};
-    const float4 q4 = q[i];
-    V3<double> qd{q4.x, q4.y, q4.z};
+    V3<double> qd;
+    if constexpr (FORM == 2) {
+      const double4 d4 = qdp[i];
+      qd = V3<double>{d4.x, d4.y, d4.z};
+    } else {
+      const float4 q4 = q[i];
+      qd = V3<double>{q4.x, q4.y, q4.z};
+    }
     float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
     Counters ct{0, 0, 0, 0, 0};
     for (uint32_t it = 0; it < nIter; ++it) {
       typename FormT<FORM>::Acc acc;
       acc.init();
       const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
-      const V3<T> qt{qf.x, qf.y, qf.z};
-      traverse<FORM, SEM, KT, false>(acc, w, tv, qt, (T)cutoff, minDepth, st, tid, ct);
+      V3<T> qt;
+      if constexpr (FORM == 2)
+        qt = V3<T>{qd.x, qd.y, qd.z};
+      else
+        qt = V3<T>{qf.x, qf.y, qf.z};
+      traverse<FORM, SEM, KT, false>(acc, kp, tv, qt, (T)cutoff, minDepth, st, tid, ct);
       fit_step<FORM, SEM>(acc, qd, qf, stepMax, nout);
     }
     const float4 p4 = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
     q[i] = p4;
     qn[i] = nout;
+    if constexpr (FORM == 2) qdp[i] = make_double4(qd.x, qd.y, qd.z, 0.0);
     if (outOff != kNoOut) {
       const uint64_t d = outOff + bo.pidx[i], e = (uint64_t)bo.cap + d;
       bo.out[3 * d] = p4.x;
@@ -765,9 +781,9 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
 }
 
 template <int FORM, int SEM>
-void launch_batch_all(const ReqDesc* r, const uint2* blocks, uint32_t nblk, float4* q, float4* qn, const BatchOut& bo,
-                      uint32_t prio, hipStream_t s) {
-  k_project_batch<FORM, SEM, -1><<<nblk, PBS, 0, s>>>(r, blocks, q, qn, bo, prio);
+void launch_batch_all(const ReqDesc* r, const uint2* blocks, uint32_t nblk, float4* q, float4* qn, double4* qd,
+                      const BatchOut& bo, uint32_t prio, hipStream_t s) {
+  k_project_batch<FORM, SEM, -1><<<nblk, PBS, 0, s>>>(r, blocks, q, qn, qd, bo, prio);
 }
 
 template <int FORM, int SEM, int KT, bool CNT>
@@ -831,34 +847,21 @@ KParams make_kparams(const mlod_kernel& k) {
   return p;
 }
 
-WCoef make_wcoef(const mlod_kernel& k) {
-  const KParams p = make_kparams(k);
-  WCoef w{};
-  w.type = p.type;
-  w.ns = p.ns;
-  w.eint = p.eint;
-  w.c = p.c;
-  w.cp = p.cp;
-  w.e = p.e;
-  for (int s = 0; s < kServeMaxSigmas; ++s) {
-    w.a[s] = p.a[s];
-    w.b[s] = p.b[s];
-  }
-  return w;
-}
-
 int launch_project_batch(const ReqDesc* d_reqs, const uint2* d_blocks, uint32_t nblk, float4* d_q, float4* d_qn,
-                         const BatchOut& bo, uint32_t prio, int semantics, int precision, hipStream_t s) {
+                         double4* d_qd, const BatchOut& bo, uint32_t prio, int semantics, int precision,
+                         hipStream_t s) {
   if (nblk == 0) return MLOD_OK;
   const bool cpu = semantics == MLOD_SEM_REF_CPU;
   if (precision == MLOD_PREC_FP32C) {
-    if (cpu) launch_batch_all<1, MLOD_SEM_REF_CPU>(d_reqs, d_blocks, nblk, d_q, d_qn, bo, prio, s);
-    else launch_batch_all<1, MLOD_SEM_REF_GPU>(d_reqs, d_blocks, nblk, d_q, d_qn, bo, prio, s);
+    if (cpu) launch_batch_all<1, MLOD_SEM_REF_CPU>(d_reqs, d_blocks, nblk, d_q, d_qn, d_qd, bo, prio, s);
+    else launch_batch_all<1, MLOD_SEM_REF_GPU>(d_reqs, d_blocks, nblk, d_q, d_qn, d_qd, bo, prio, s);
   } else if (precision == MLOD_PREC_FP32) {
-    if (cpu) launch_batch_all<0, MLOD_SEM_REF_CPU>(d_reqs, d_blocks, nblk, d_q, d_qn, bo, prio, s);
-    else launch_batch_all<0, MLOD_SEM_REF_GPU>(d_reqs, d_blocks, nblk, d_q, d_qn, bo, prio, s);
+    if (cpu) launch_batch_all<0, MLOD_SEM_REF_CPU>(d_reqs, d_blocks, nblk, d_q, d_qn, d_qd, bo, prio, s);
+    else launch_batch_all<0, MLOD_SEM_REF_GPU>(d_reqs, d_blocks, nblk, d_q, d_qn, d_qd, bo, prio, s);
   } else {
-    return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
+    if (!d_qd) return fail(MLOD_ERR_INVALID, "the FP64 form needs the FP64 position pool");
+    if (cpu) launch_batch_all<2, MLOD_SEM_REF_CPU>(d_reqs, d_blocks, nblk, d_q, d_qn, d_qd, bo, prio, s);
+    else launch_batch_all<2, MLOD_SEM_REF_GPU>(d_reqs, d_blocks, nblk, d_q, d_qn, d_qd, bo, prio, s);
   }
   LAUNCH_CHECK();
   return MLOD_OK;

@@ -27,12 +27,24 @@
 //   worker polls the flags and copies a run to its caller as soon as the run is done, without
 //   waiting for the rest of its launch.
 //
+// * Admission by memory footprint.  A run holds pool blocks (kBatchWG query slots of 36 B, 68 B
+//   in the FP64 form) from its first launch to its hand-back; chunks are admitted while the
+//   pool has blocks for them and wait in their queue otherwise.  mlod_server_set_budget caps the
+//   HBM the server holds with its trees (resident trees + staging + pool): the pool is resized to
+//   what the budget leaves, trees that would exceed it are refused, and so are requests whose
+//   chunks could never be resident under it.
+// * Cancellation.  mlod_server_cancel drops a request's queued chunks and resident runs at the
+//   next launch boundary; runs already in a launch finish it and are discarded.  The request
+//   completes with status 1 and its output buffers hold whatever runs were handed back before.
+//
 // A query gives the bits of a standalone projection whatever the mix and the split of its steps
 // over launches: a lane's traversal and fit depend only on its query and tree, and the position
-// carried from one step to the next is the float the next step starts from.  Only the queues,
-// the tree list and the completion list are shared with callers (under the mutex); the pool,
-// runs and streams belong to the worker thread, and staging copies and launches run outside the
-// mutex.
+// carried from one step to the next is the one the next step starts from (the float position,
+// or the FP64 one in the FP64 form).  A request's tree is resolved when it is submitted.  Only the
+// queues, the tree list, the ticket table and the completion list are shared with callers (under
+// the mutex); the pool, runs and streams belong to the worker thread, and staging copies and
+// launches run outside the mutex.
+#include <cmath>
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -98,25 +110,29 @@ __global__ __launch_bounds__(BS) void k_batch_keys(const float* __restrict__ in,
   idx[i] = i;
 }
 
-// sorted staged query j -> its pool slot, recording its index within the run
+// sorted staged query j -> its pool slot, recording its index within the run (and, in the FP64
+// form, its FP64 position)
 __global__ __launch_bounds__(BS) void k_pool_gather(const float* __restrict__ in, const uint64_t* __restrict__ key,
                                                     const uint32_t* __restrict__ perm, uint32_t slots,
                                                     const ReqDesc* __restrict__ reqs, float4* __restrict__ q,
-                                                    float4* __restrict__ qn, uint32_t* __restrict__ pidx) {
+                                                    float4* __restrict__ qn, double4* __restrict__ qd,
+                                                    uint32_t* __restrict__ pidx) {
   const uint32_t j = blockIdx.x * BS + threadIdx.x;
   if (j >= slots) return;
   const uint32_t s = perm[j];
   const ReqDesc& R = reqs[(uint32_t)(key[j] >> 30)];
   const uint32_t d = R.pad0 + (j - R.slot0);
-  q[d] = make_float4(in[3 * (uint64_t)s], in[3 * (uint64_t)s + 1], in[3 * (uint64_t)s + 2], 0.f);
+  const float x = in[3 * (uint64_t)s], y = in[3 * (uint64_t)s + 1], z = in[3 * (uint64_t)s + 2];
+  q[d] = make_float4(x, y, z, 0.f);
   qn[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (qd) qd[d] = make_double4(x, y, z, 0.0);
   pidx[d] = s - R.slot0;
 }
 
 // One request as submitted.
 struct Request {
   uint64_t ticket;
-  uint32_t tree;
+  const mlod_tree* tree;  // resolved at submission
   mlod_kernel kernel;
   float cutoff;
   uint32_t minDepth;
@@ -129,6 +145,7 @@ struct Request {
   uint32_t left;  // queries not yet handed back (worker thread only)
   bool interactive;
   bool started;
+  std::atomic<bool> cancelled{false};
   double deadline;  // ms since server creation
   clk::time_point submitted, launched;
 };
@@ -200,7 +217,15 @@ struct mlod_server {
   int precision = MLOD_PREC_FP32C;
   int device = 0;
   std::vector<mlod_tree*> trees;
+  std::map<uint64_t, mlod::Request*> live;  // outstanding requests by ticket
   uint32_t chunk = 0;
+  // memory footprint (bytes of device memory): budget (0 = none), registered trees, staging of
+  // all streams, pool; a budget change is applied by the worker when it is idle
+  uint64_t budget = 0, treeBytes = 0, stageBytes = 0, poolBytes = 0;
+  uint64_t budgetReq = 0;
+  bool budgetPending = false;
+  int budgetResult = MLOD_OK;
+  std::string budgetMsg;
   mlod::ChunkQueue qInt, qBulk;  // queued chunks per class
   uint64_t nextSeq = 0;
   std::vector<mlod_server_done> done;
@@ -219,6 +244,7 @@ struct mlod_server {
   std::vector<mlod::Run*> runs;  // resident runs
   float4* dQ = nullptr;          // query pool
   float4* dQn = nullptr;
+  double4* dQd = nullptr;        // FP64 positions (FP64 form only)
   uint32_t* dPidx = nullptr;
   uint32_t poolBlocks = 0, freeBlocks = 0;
   std::map<uint32_t, uint32_t> freeRanges;  // first block -> length
@@ -253,6 +279,18 @@ void free_stream(Stream& b) {
   b = Stream{};
 }
 
+// device bytes of one stream's staging and launch tables
+uint64_t stream_device_bytes(uint32_t cap, size_t sortBytes) {
+  const uint64_t nblk = max_launch_blocks(cap);
+  return 3ull * cap * sizeof(float) + 2ull * cap * sizeof(uint64_t) + 2ull * cap * sizeof(uint32_t) +
+         (uint64_t)kMaxRuns * sizeof(ReqDesc) + nblk * sizeof(uint2) + sortBytes + 16;
+}
+
+// device bytes of one pool block
+uint64_t block_bytes(int precision) {
+  return (uint64_t)kBatchWG * (2 * sizeof(float4) + sizeof(uint32_t) + (precision == MLOD_PREC_FP64 ? sizeof(double4) : 0));
+}
+
 hipError_t alloc_stream(Stream& b, uint32_t cap) {
   hipError_t e;
   const size_t nblk = max_launch_blocks(cap);
@@ -275,22 +313,32 @@ hipError_t alloc_stream(Stream& b, uint32_t cap) {
 }
 
 void free_pool(mlod_server* S) {
-  void* d[] = {S->dQ, S->dQn, S->dPidx};
+  void* d[] = {S->dQ, S->dQn, S->dQd, S->dPidx};
   for (void* p : d)
     if (p) (void)hipFree(p);
   S->dQ = S->dQn = nullptr;
+  S->dQd = nullptr;
   S->dPidx = nullptr;
+  S->poolBytes = 0;
+  S->poolBlocks = S->freeBlocks = 0;
+  S->freeRanges.clear();
 }
 
 hipError_t alloc_pool(mlod_server* S, uint32_t blocks) {
   hipError_t e;
   const size_t slots = (size_t)blocks * kBatchWG;
   if ((e = dmalloc(&S->dQ, slots)) || (e = dmalloc(&S->dQn, slots)) || (e = dmalloc(&S->dPidx, slots))) return e;
+  if (S->precision == MLOD_PREC_FP64 && (e = dmalloc(&S->dQd, slots))) return e;
   S->poolBlocks = S->freeBlocks = blocks;
+  S->poolBytes = blocks * block_bytes(S->precision);
   S->freeRanges.clear();
   S->freeRanges[0] = blocks;
   return hipSuccess;
 }
+
+// Blocks a bulk run must leave free for interactive ones: one launch's worth, at most half the
+// pool (so a small, budgeted pool still admits bulk chunks).
+uint32_t bulk_reserve(const mlod_server* S) { return std::min(max_launch_blocks(S->cap), S->poolBlocks / 2); }
 
 // First fit; false when no free range is long enough.
 bool pool_alloc(mlod_server* S, uint32_t n, uint32_t& blk0) {
@@ -349,26 +397,32 @@ struct Cand {
 
 // Fills st.runs with the runs of the stream's next launch: candidates of class cls, most urgent
 // first, at most S->cap queries; queued chunks become new runs (consecutive chunks of a request
-// one run) when pool blocks are free for them.  Returns the queries picked; the sequence numbers
-// of the chunks placed go to `placed`.
+// one run) while the pool has free blocks for them (bulk chunks leave bulk_reserve blocks to
+// interactive ones); the others stay queued.  Returns the queries picked; the sequence numbers of
+// the chunks placed go to `placed`.
 uint32_t take_class(mlod_server* S, Stream& st, const std::vector<Cand>& cands, int cls, std::vector<uint64_t>& placed) {
   const uint32_t steps = cls == 0 ? kInteractiveSteps : kBulkSteps;
-  // bulk chunks leave one launch's worth of pool blocks to interactive ones
-  const uint32_t reserve = cls == 0 ? 0u : max_launch_blocks(S->cap);
+  const uint32_t reserve = cls == 0 ? 0u : bulk_reserve(S);
+  uint32_t avail = S->freeBlocks > reserve ? S->freeBlocks - reserve : 0u;
   uint32_t used = 0;
   std::vector<Chunk> taken;
   for (const Cand& c : cands) {
     const uint32_t m = c.run ? c.run->m : c.ch.m;
     if (c.u.cls != cls || used + m > S->cap || st.runs.size() >= kMaxRuns) continue;
-    used += m;
     if (c.run) {
+      used += m;
       c.run->steps = std::min(steps, c.run->stepsLeft);
       st.runs.push_back(c.run);
       continue;
     }
-    taken.push_back(c.ch);
     Run* last = st.runs.empty() ? nullptr : st.runs.back();
-    if (last && last->fresh && last->req == c.ch.req && last->off + last->m == c.ch.off) {
+    const bool merge = last && last->fresh && last->req == c.ch.req && last->off + last->m == c.ch.off;
+    const uint32_t nb = merge ? blocks_of(last->m + m) - blocks_of(last->m) : blocks_of(m);
+    if (nb > avail) continue;  // not admitted now: stays queued
+    avail -= nb;
+    used += m;
+    taken.push_back(c.ch);
+    if (merge) {
       last->m += m;
       continue;
     }
@@ -376,14 +430,14 @@ uint32_t take_class(mlod_server* S, Stream& st, const std::vector<Cand>& cands, 
     r->steps = std::min(steps, r->stepsLeft);
     st.runs.push_back(r);
   }
-  // pool blocks for the new runs; a run without them stays queued as its chunks
+  // pool blocks for the new runs; a run without a free range long enough stays queued as its chunks
   std::vector<Run*> keep;
   uint32_t picked = 0;
   for (Run* r : st.runs) {
     if (r->fresh) {
       const uint32_t nb = blocks_of(r->m);
       uint32_t b0 = 0;
-      if (S->freeBlocks < nb + reserve || !pool_alloc(S, nb, b0)) {
+      if (!pool_alloc(S, nb, b0)) {
         delete r;
         continue;
       }
@@ -400,10 +454,49 @@ uint32_t take_class(mlod_server* S, Stream& st, const std::vector<Cand>& cands, 
   return picked;
 }
 
+// Drops the cancelled work the worker holds outside launches: queued chunks (from `cands`) and
+// resident runs waiting in the pool.  Their queries count as handed back; a cancelled request
+// whose queries are all accounted for completes (status 1).
+void drop_cancelled(mlod_server* S, std::vector<Cand>& cands, std::vector<mlod_server_done>& out,
+                    std::vector<Request*>& finished) {
+  const auto now = clk::now();
+  auto account = [&](Request* q, uint32_t m) {
+    q->left -= m;
+    if (q->left == 0) {
+      mlod_server_done d{};
+      d.ticket = q->ticket;
+      d.latency_ms = std::chrono::duration<double, std::milli>(now - q->submitted).count();
+      d.queue_ms = std::chrono::duration<double, std::milli>(q->launched - q->submitted).count();
+      d.completed_s = std::chrono::duration<double>(now - S->t0).count();
+      d.status = 1;
+      out.push_back(d);
+      finished.push_back(q);
+    }
+  };
+  std::vector<Cand> keep;
+  for (Cand& c : cands) {
+    Request* q = c.run ? c.run->req : c.ch.req;
+    if (!q->cancelled.load(std::memory_order_relaxed)) {
+      keep.push_back(c);
+      continue;
+    }
+    if (c.run) {
+      pool_free(S, c.run->blk0, c.run->nblk);
+      S->runs.erase(std::find(S->runs.begin(), S->runs.end(), c.run));
+      account(q, c.run->m);
+      delete c.run;
+    } else {
+      account(q, c.ch.m);
+    }
+  }
+  cands.swap(keep);
+}
+
 // Picks the next launch of stream st; returns its queries (0: nothing to launch).  Interactive
 // work goes first, except that bulk work waiting for more than kBulkMaxWaitMs since the last
 // bulk launch gets the launch, so a stream of interactive requests cannot starve it.
-uint32_t pick_runs(mlod_server* S, Stream& st, bool& interactive) {
+uint32_t pick_runs(mlod_server* S, Stream& st, bool& interactive, std::vector<mlod_server_done>& dropped,
+                   std::vector<Request*>& finished) {
   const double now = ms_since(S, clk::now());
   std::vector<Cand> cands;
   for (Run* r : S->runs)
@@ -420,6 +513,7 @@ uint32_t pick_runs(mlod_server* S, Stream& st, bool& interactive) {
       }
     }
   }
+  drop_cancelled(S, cands, dropped, finished);
   if (cands.empty()) return 0;
   std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.u < b.u; });
   int cls = cands[0].u.cls;
@@ -457,10 +551,10 @@ int launch_step(mlod_server* S, Stream& st, bool interactive) {
     Run* r = st.runs[k];
     r->stream = sid;
     Request& q = *r->req;
-    const mlod_tree* t = S->trees[q.tree];
+    const mlod_tree* t = q.tree;
     ReqDesc& d = st.hReq[k];
     d.tv = t->view();
-    d.w = make_wcoef(q.kernel);
+    d.kp = make_kparams(q.kernel);
     d.cutoff = q.cutoff;
     d.stepMax = q.stepMax < 0.f ? t->radius : q.stepMax;
     d.minDepth = q.minDepth;
@@ -506,7 +600,8 @@ int launch_step(mlod_server* S, Stream& st, bool interactive) {
       e = rocprim::radix_sort_pairs(st.sortTmp, st.sortBytes, st.dKey, st.dKey2, st.dIdx, st.dPerm, staged, 0,
                                     30 + runBits, st.s);
     if (!e) {
-      k_pool_gather<<<grid, BS, 0, st.s>>>(st.dIn, st.dKey2, st.dPerm, staged, st.dReq, S->dQ, S->dQn, S->dPidx);
+      k_pool_gather<<<grid, BS, 0, st.s>>>(st.dIn, st.dKey2, st.dPerm, staged, st.dReq, S->dQ, S->dQn, S->dQd,
+                                           S->dPidx);
       e = hipGetLastError();
     }
   }
@@ -514,8 +609,8 @@ int launch_step(mlod_server* S, Stream& st, bool interactive) {
   for (Run* r : st.runs) r->fresh = false;
   st.seq += 1;
   const BatchOut bo{S->dPidx, st.dOut, S->cap, st.dFlag, st.seq};
-  if (int rc = launch_project_batch(st.dReq, st.dBlk, nBlk, S->dQ, S->dQn, bo, interactive ? 1u : 0u, S->semantics,
-                                    S->precision, st.s))
+  if (int rc = launch_project_batch(st.dReq, st.dBlk, nBlk, S->dQ, S->dQn, S->dQd, bo, interactive ? 1u : 0u,
+                                    S->semantics, S->precision, st.s))
     return rc;
   e = hipEventRecord(st.ev, st.s);
   if (e) return hip_fail(e, "serving launch", __FILE__, __LINE__);
@@ -529,16 +624,19 @@ uint64_t hand_back(mlod_server* S, Stream& st, size_t k, clk::time_point now, st
   Run* r = st.runs[k];
   Request& q = *r->req;
   const size_t o = st.outOff[k];
-  if (q.outP) std::memcpy(q.outP + 3 * (size_t)r->off, st.hOut + 3 * o, (size_t)r->m * 3 * sizeof(float));
-  if (q.outN)
+  const bool cancelled = q.cancelled.load(std::memory_order_relaxed);
+  if (q.outP && !cancelled)
+    std::memcpy(q.outP + 3 * (size_t)r->off, st.hOut + 3 * o, (size_t)r->m * 3 * sizeof(float));
+  if (q.outN && !cancelled)
     std::memcpy(q.outN + 3 * (size_t)r->off, st.hOut + 3 * ((size_t)S->cap + o), (size_t)r->m * 3 * sizeof(float));
   pool_free(S, r->blk0, r->nblk);
   S->runs.erase(std::find(S->runs.begin(), S->runs.end(), r));
   r->handed = true;
   q.left -= r->m;
   if (q.left == 0) {
-    mlod_server_done d;
+    mlod_server_done d{};
     d.ticket = q.ticket;
+    d.status = cancelled ? 1 : 0;
     d.latency_ms = std::chrono::duration<double, std::milli>(now - q.submitted).count();
     d.queue_ms = std::chrono::duration<double, std::milli>(q.launched - q.submitted).count();
     d.completed_s = std::chrono::duration<double>(now - S->t0).count();
@@ -573,6 +671,11 @@ uint64_t complete_step(mlod_server* S, Stream& st, std::vector<mlod_server_done>
     Run* r = st.runs[k];
     r->stepsLeft -= r->steps;
     r->stream = -1;
+    if (st.outOff[k] == kNoOut && r->req->cancelled.load(std::memory_order_relaxed)) {
+      st.outOff[k] = 0;  // discarded through hand_back, which copies nothing for a cancelled request
+      handed += hand_back(S, st, k, now, out, finished);
+      continue;
+    }
     if (st.outOff[k] == kNoOut) continue;
     if (!r->handed) handed += hand_back(S, st, k, now, out, finished);
   }
@@ -591,6 +694,62 @@ void set_error(mlod_server* S, int code, const std::string& msg) {
   S->cvDone.notify_all();
 }
 
+// Publishes completed requests to callers (worker thread): done records, counters, and the
+// requests leave the ticket table.
+void publish(mlod_server* S, std::vector<mlod_server_done>& fin, std::vector<Request*>& finished, uint64_t handed) {
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->done.insert(S->done.end(), fin.begin(), fin.end());
+    S->requests += fin.size();
+    S->queries += handed;
+    for (Request* r : finished) S->live.erase(r->ticket);
+  }
+  for (Request* r : finished) delete r;
+  if (!fin.empty()) S->cvDone.notify_all();
+  fin.clear();
+  finished.clear();
+}
+
+// Applies a pending budget (worker thread, with nothing resident and no launch in flight): the
+// pool is reallocated to the blocks the budget leaves after the trees and the staging, at most
+// the default pool, at least two chunks' worth.
+void apply_budget(mlod_server* S) {
+  const uint64_t budget = S->budgetReq;
+  const uint32_t defBlocks = kPoolLaunches * (uint32_t)S->streams.size() * max_launch_blocks(S->cap);
+  uint32_t blocks = defBlocks;
+  int rc = MLOD_OK;
+  std::string msg;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (budget) {
+      const uint64_t fixed = S->treeBytes + S->stageBytes;
+      const uint64_t left = budget > fixed ? budget - fixed : 0;
+      blocks = (uint32_t)std::min<uint64_t>(defBlocks, left / block_bytes(S->precision));
+      if (blocks < 2 * blocks_of(S->chunk)) {
+        rc = MLOD_ERR_OOM;
+        msg = "admission: the budget of " + std::to_string(budget) + " bytes leaves " + std::to_string(left) +
+              " bytes after the trees (" + std::to_string(S->treeBytes) + ") and staging (" +
+              std::to_string(S->stageBytes) + "): less than two chunks of pool";
+      }
+    }
+  }
+  if (rc == MLOD_OK && blocks != S->poolBlocks) {
+    free_pool(S);
+    if (hipError_t e = alloc_pool(S, blocks)) {
+      rc = MLOD_ERR_OOM;
+      msg = std::string("admission: pool allocation failed: ") + hipGetErrorString(e);
+      free_pool(S);
+      (void)alloc_pool(S, std::min(blocks, 2 * blocks_of(S->chunk)));
+    }
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (rc == MLOD_OK) S->budget = budget;
+  S->budgetResult = rc;
+  S->budgetMsg = msg;
+  S->budgetPending = false;
+  S->cvDone.notify_all();
+}
+
 void worker_main(mlod_server* S) {
   (void)hipSetDevice(S->device);
   std::vector<mlod_server_done> fin;
@@ -601,8 +760,6 @@ void worker_main(mlod_server* S) {
     for (Stream& st : S->streams) {
       if (!st.busy) continue;
       const hipError_t q = hipEventQuery(st.ev);
-      fin.clear();
-      finished.clear();
       uint64_t handed = 0;
       if (q == hipErrorNotReady) handed = poll_runs(S, st, fin, finished);
       if (q == hipSuccess || handed) {
@@ -612,14 +769,7 @@ void worker_main(mlod_server* S) {
         } else {
           anyBusy = true;
         }
-        for (Request* r : finished) delete r;
-        {
-          std::lock_guard<std::mutex> lk(S->mu);
-          S->done.insert(S->done.end(), fin.begin(), fin.end());
-          S->requests += fin.size();
-          S->queries += handed;
-        }
-        if (!fin.empty()) S->cvDone.notify_all();
+        publish(S, fin, finished, handed);
         progressed = true;
       } else if (q != hipErrorNotReady) {
         set_error(S, MLOD_ERR_HIP, std::string("serving launch failed: ") + hipGetErrorString(q));
@@ -628,11 +778,26 @@ void worker_main(mlod_server* S) {
         anyBusy = true;
       }
     }
+    // a budget change waits until nothing is resident and no launch is in flight
+    bool budget = false;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      budget = S->budgetPending;
+    }
+    if (budget && !anyBusy && S->runs.empty()) {
+      apply_budget(S);
+      progressed = true;
+    }
     // next launches on idle streams
     for (Stream& st : S->streams) {
-      if (st.busy) continue;
+      if (st.busy || budget) continue;
       bool interactive = false;
-      if (pick_runs(S, st, interactive) == 0) break;
+      const uint32_t picked = pick_runs(S, st, interactive, fin, finished);
+      if (!fin.empty()) {
+        publish(S, fin, finished, 0);
+        progressed = true;
+      }
+      if (picked == 0) break;
       const int r = launch_step(S, st, interactive);
       if (r) {
         set_error(S, r, g_last_error());
@@ -653,7 +818,7 @@ void worker_main(mlod_server* S) {
       std::this_thread::yield();
     } else {
       S->cv.wait_for(lk, std::chrono::milliseconds(1),
-                     [S] { return S->stop || !S->qInt.empty() || !S->qBulk.empty(); });
+                     [S] { return S->stop || S->budgetPending || !S->qInt.empty() || !S->qBulk.empty(); });
     }
   }
 }
@@ -672,8 +837,7 @@ int mlod_server_create(uint32_t max_batch_queries, uint32_t chunk_queries, uint3
   if (max_batch_queries < kBatchWG || max_batch_queries > (1u << 24) || inflight == 0 || inflight > 8)
     return fail(MLOD_ERR_INVALID, "bad launch capacity or inflight");
   if (chunk_queries > max_batch_queries) return fail(MLOD_ERR_INVALID, "chunk larger than a launch");
-  if (precision != MLOD_PREC_FP32 && precision != MLOD_PREC_FP32C)
-    return fail(MLOD_ERR_UNSUPPORTED, "serving runs the FP32 and FP32C forms");
+  if (precision < MLOD_PREC_FP32 || precision > MLOD_PREC_FP64) return fail(MLOD_ERR_INVALID, "bad precision");
   if (semantics != MLOD_SEM_REF_GPU && semantics != MLOD_SEM_REF_CPU) return fail(MLOD_ERR_INVALID, "bad semantics");
   mlod_server* S = new mlod_server();
   S->cap = max_batch_queries;
@@ -683,7 +847,10 @@ int mlod_server_create(uint32_t max_batch_queries, uint32_t chunk_queries, uint3
   (void)hipGetDevice(&S->device);
   S->streams.resize(inflight);
   hipError_t e = alloc_pool(S, kPoolLaunches * inflight * max_launch_blocks(S->cap));
-  for (size_t i = 0; !e && i < S->streams.size(); ++i) e = alloc_stream(S->streams[i], S->cap);
+  for (size_t i = 0; !e && i < S->streams.size(); ++i) {
+    e = alloc_stream(S->streams[i], S->cap);
+    if (!e) S->stageBytes += stream_device_bytes(S->cap, S->streams[i].sortBytes);
+  }
   if (e) {
     for (Stream& c : S->streams) free_stream(c);
     free_pool(S);
@@ -720,11 +887,44 @@ int mlod_server_destroy(mlod_server* S) {
   return MLOD_OK;
 }
 
+int mlod_server_set_budget(mlod_server* S, uint64_t hbm_bytes) {
+  if (!S) return fail(MLOD_ERR_INVALID, "NULL argument");
+  std::unique_lock<std::mutex> lk(S->mu);
+  if (S->error) return fail(S->error, S->errmsg);
+  if (!S->live.empty()) return fail(MLOD_ERR_STATE, "the budget changes only while no request is outstanding");
+  if (hbm_bytes && S->treeBytes + S->stageBytes > hbm_bytes)
+    return fail(MLOD_ERR_OOM, "admission: the registered trees (" + std::to_string(S->treeBytes) +
+                                  " bytes) and staging already exceed the budget");
+  S->budgetReq = hbm_bytes;
+  S->budgetPending = true;
+  S->cv.notify_all();
+  S->cvDone.wait(lk, [S] { return !S->budgetPending || S->error; });
+  if (S->error) return fail(S->error, S->errmsg);
+  if (S->budgetResult) return fail(S->budgetResult, S->budgetMsg);
+  return MLOD_OK;
+}
+
+int mlod_server_memory(mlod_server* S, uint64_t* out5) {
+  if (!S || !out5) return fail(MLOD_ERR_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  out5[0] = S->budget;
+  out5[1] = S->treeBytes;
+  out5[2] = S->stageBytes;
+  out5[3] = S->poolBytes;
+  out5[4] = (uint64_t)S->poolBlocks * kBatchWG;
+  return MLOD_OK;
+}
+
 int mlod_server_add_tree(mlod_server* S, mlod_tree* t, uint32_t* tree_id) {
   if (!S || !t || !tree_id) return fail(MLOD_ERR_INVALID, "NULL argument");
   std::lock_guard<std::mutex> lk(S->mu);
+  if (S->budget && S->treeBytes + t->bytes + S->stageBytes + S->poolBytes > S->budget)
+    return fail(MLOD_ERR_OOM, "admission: a tree of " + std::to_string(t->bytes) + " bytes exceeds the budget of " +
+                                  std::to_string(S->budget) + " bytes (trees " + std::to_string(S->treeBytes) +
+                                  ", staging and pool " + std::to_string(S->stageBytes + S->poolBytes) + ")");
   *tree_id = (uint32_t)S->trees.size();
   S->trees.push_back(t);
+  S->treeBytes += t->bytes;
   return MLOD_OK;
 }
 
@@ -737,11 +937,10 @@ int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, f
     if (S->error) return fail(S->error, S->errmsg);
   }
   if (m == 0) return fail(MLOD_ERR_INVALID, "empty request");
-  if (k->type < 0 || k->type > 2) return fail(MLOD_ERR_INVALID, "bad kernel type");
-  if (k->type == MLOD_KERNEL_GAUSSIAN_MULTIPLE && (k->n_sigmas < 1 || k->n_sigmas > kServeMaxSigmas))
-    return fail(MLOD_ERR_UNSUPPORTED, "serving takes 1..8 sigmas");
+  if (int rc = validate_kernel(k)) return rc;
+  if (!(cutoff >= 0.f)) return fail(MLOD_ERR_INVALID, "cutoff must be >= 0");
+  if (std::isnan(step_max)) return fail(MLOD_ERR_INVALID, "step_max is NaN");
   Request* r = new Request();
-  r->tree = tree_id;
   r->kernel = *k;
   r->cutoff = cutoff;
   r->minDepth = min_depth;
@@ -761,6 +960,12 @@ int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, f
       delete r;
       return fail(MLOD_ERR_INVALID, "unknown tree id");
     }
+    // the pool must hold at least one chunk of it as a bulk run
+    if (blocks_of(std::min(m, S->chunk)) + bulk_reserve(S) > S->poolBlocks) {
+      delete r;
+      return fail(MLOD_ERR_OOM, "admission: a chunk of this request can never be resident under the budget");
+    }
+    r->tree = S->trees[tree_id];
     r->ticket = S->nextTicket++;
     r->submitted = clk::now();
     r->launched = r->submitted;
@@ -769,7 +974,7 @@ int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, f
       // no step: the queries come back unchanged with zero normals, as from a standalone projection
       if (out_pts) std::memcpy(out_pts, queries, (size_t)m * 3 * sizeof(float));
       if (out_nmls) std::memset(out_nmls, 0, (size_t)m * 3 * sizeof(float));
-      mlod_server_done d;
+      mlod_server_done d{};
       d.ticket = r->ticket;
       d.latency_ms = d.queue_ms = 0.0;
       d.completed_s = std::chrono::duration<double>(clk::now() - S->t0).count();
@@ -781,9 +986,20 @@ int mlod_server_submit(mlod_server* S, uint32_t tree_id, const mlod_kernel* k, f
       return MLOD_OK;
     }
     r->deadline = ms_since(S, r->submitted) + pts / kNominalPtsPerMs;
+    S->live[r->ticket] = r;
     ChunkQueue& Q = r->interactive ? S->qInt : S->qBulk;
     for (uint32_t off = 0; off < m; off += S->chunk) Q.push(Chunk{r, off, std::min(S->chunk, m - off), S->nextSeq++});
   }
+  S->cv.notify_one();
+  return MLOD_OK;
+}
+
+int mlod_server_cancel(mlod_server* S, uint64_t ticket) {
+  if (!S) return fail(MLOD_ERR_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->live.find(ticket);
+  if (it == S->live.end()) return fail(MLOD_ERR_INVALID, "no outstanding request with this ticket");
+  it->second->cancelled.store(true, std::memory_order_relaxed);
   S->cv.notify_one();
   return MLOD_OK;
 }

@@ -32,7 +32,8 @@ class Server:
     """A serving instance: resident trees, a worker thread and ``inflight`` streams, each with
     one step launch of at most ``max_batch`` queries on the GPU at a time; requests are cut into
     chunks of ``chunk`` queries (0: max_batch / 4) that stay in a device pool between steps.
-    ``stats()['batches']`` counts step launches."""
+    ``stats()['batches']`` counts step launches.  ``submit`` and ``poll`` may run on different
+    threads: a request's buffers are recorded under a lock before its ticket can complete."""
 
     def __init__(self, max_batch=131072, inflight=3, semantics="ref_gpu", precision="fp32c", chunk=0):
         sem = {"ref_gpu": 0, "ref_cpu": 1}[semantics]
@@ -43,12 +44,25 @@ class Server:
         self.max_batch = int(max_batch)
         self._keep = {}
         self._trees = []
+        self._lock = threading.Lock()
 
     def add_tree(self, tree: APSSOctree) -> int:
         tid = C.c_uint32()
         check(lib().mlod_server_add_tree(self.h, tree.handle, C.byref(tid)))
         self._trees.append(tree)
         return int(tid.value)
+
+    def set_budget(self, hbm_bytes: int):
+        """HBM budget over the registered trees, staging and query pool (0: none)."""
+        check(lib().mlod_server_set_budget(self.h, int(hbm_bytes)))
+
+    def memory(self):
+        o = (C.c_uint64 * 5)()
+        check(lib().mlod_server_memory(self.h, o))
+        return dict(budget=o[0], trees=o[1], staging=o[2], pool=o[3], pool_slots=o[4])
+
+    def cancel(self, ticket: int):
+        check(lib().mlod_server_cancel(self.h, int(ticket)))
 
     def submit(self, tree_id, queries, kernel: Kernel, cutoff=2.0, min_depth=0, n_iter=10, step_max=-1.0, out=True):
         q = np.ascontiguousarray(queries, np.float32)
@@ -57,23 +71,29 @@ class Server:
         on = np.empty((m, 3), np.float32) if out else None
         t = C.c_uint64()
         kc = kernel.to_c()
-        check(lib().mlod_server_submit(self.h, int(tree_id), C.byref(kc), float(cutoff), int(min_depth), int(n_iter),
-                                       float(step_max), q.ctypes.data_as(C.c_void_p), m,
-                                       op.ctypes.data_as(C.c_void_p) if out else None,
-                                       on.ctypes.data_as(C.c_void_p) if out else None, C.byref(t)))
-        self._keep[t.value] = (q, op, on)
+        # the ticket can complete (and be polled on another thread) as soon as the call returns,
+        # so its buffers are recorded under the lock the poller takes
+        with self._lock:
+            check(lib().mlod_server_submit(self.h, int(tree_id), C.byref(kc), float(cutoff), int(min_depth),
+                                           int(n_iter), float(step_max), q.ctypes.data_as(C.c_void_p), m,
+                                           op.ctypes.data_as(C.c_void_p) if out else None,
+                                           on.ctypes.data_as(C.c_void_p) if out else None, C.byref(t)))
+            self._keep[t.value] = (q, op, on)
         return int(t.value)
 
-    def poll(self, timeout=0.0, cap=4096):
-        """Completed requests: list of (ticket, latency_ms, queue_ms, completed_s, points, normals)."""
+    def poll(self, timeout=0.0, cap=4096, with_status=False):
+        """Completed requests: list of (ticket, latency_ms, queue_ms, completed_s, points, normals)
+        (plus the status, 0 completed or 1 cancelled, with ``with_status``)."""
         buf = (_lib.ServerDoneC * cap)()
         n = C.c_uint32()
         check(lib().mlod_server_poll(self.h, buf, cap, C.byref(n), float(timeout)))
         out = []
-        for i in range(n.value):
-            d = buf[i]
-            q, op, on = self._keep.pop(d.ticket)
-            out.append((d.ticket, d.latency_ms, d.queue_ms, d.completed_s, op, on))
+        with self._lock:
+            for i in range(n.value):
+                d = buf[i]
+                q, op, on = self._keep.pop(d.ticket)
+                rec = (d.ticket, d.latency_ms, d.queue_ms, d.completed_s, op, on)
+                out.append(rec + (d.status,) if with_status else rec)
         return out
 
     def stats(self):
@@ -134,8 +154,10 @@ def make_kernel(name, radius):
 
 
 def run_load(server, trees, clouds, rate=2000.0, seconds=2.0, min_req=1_000, max_req=500_000, n_iter=10, cutoff=2.0,
-             seed=5, pool=500_000, max_requests=None, log=None):
-    """Poisson arrivals at `rate` req/s for `seconds`; returns per-request records and totals."""
+             seed=5, pool=500_000, max_requests=None, log=None, keep_queries=False):
+    """Poisson arrivals at `rate` req/s for `seconds`; returns the totals, the completion records
+    and, per ticket, (queries, kernel name, cloud, submit time[, the queries themselves with
+    ``keep_queries``])."""
     rng = np.random.default_rng(seed)
     tids = [server.add_tree(t) for t in trees] if not hasattr(server, "_tids") else server._tids
     server._tids = tids
@@ -177,7 +199,7 @@ def run_load(server, trees, clouds, rate=2000.0, seconds=2.0, min_req=1_000, max
         off = int(rng.integers(0, len(qp) - m + 1))
         k = make_kernel(KERNELS[kern_i[i]], radii[c])
         tk = server.submit(tids[c], qp[off:off + m], k, cutoff=cutoff, n_iter=n_iter)
-        meta[tk] = (m, KERNELS[kern_i[i]], c, time.perf_counter() - t0)
+        meta[tk] = (m, KERNELS[kern_i[i]], c, time.perf_counter() - t0) + ((qp[off:off + m],) if keep_queries else ())
     submit_end = time.perf_counter() - t0
     stop.set()
     th.join()

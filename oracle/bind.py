@@ -93,6 +93,8 @@ def oracle_lib():
         L.orc_project.argtypes = [_vp, C.POINTER(KernelC), C.POINTER(ParamsC), C.c_int32, _f, C.c_uint32, C.c_uint32,
                                   C.c_float, _d, _d, C.c_int32]
         L.orc_fit.argtypes = [_d, _d, C.c_double, C.c_int32, _d, _d]
+        L.orc_stats.restype = C.c_int32
+        L.orc_stats.argtypes = [_vp, C.POINTER(KernelC), C.POINTER(ParamsC), _f, C.c_uint32, _d]
         L.orc_gen_create.restype = _vp
         L.orc_gen_create.argtypes = [PROJ_CB, _vp]
         L.orc_gen_free.argtypes = [_vp]
@@ -217,6 +219,15 @@ class OracleTree:
                                float(step_max), _ptr(op, C.c_double), _ptr(on, C.c_double), nt)
         assert r == 0
         return op, on
+
+    def stats(self, queries, kernel, params):
+        """FP64 weighted sums (w, wp, wn, wpn, wpp) of one MLoD or global pass per query."""
+        q = _c32(queries)
+        out = np.zeros((len(q), 9), np.float64)
+        r = self.L.orc_stats(self.h, C.byref(kernel), C.byref(params), _ptr(q, C.c_float), len(q),
+                             _ptr(out, C.c_double))
+        assert r == 0
+        return out
 
 
 def oracle_weight(kernel, p, q, precision=1):

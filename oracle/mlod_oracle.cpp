@@ -19,6 +19,12 @@
 //   weights       : closed forms against numpy.
 //   MLoD traversal: golden vectors of the reference CPU path (REF_CPU semantics, FP32);
 //                   cutoff -> infinity reduces it exactly to brute-force global APSS.
+//   REF_GPU terms : the parent share a_c / a_node equals REF_CPU's 1 / #children on an
+//                   octree of equal sibling areas (a lattice), and is checked term by term
+//                   on a hand-written three-point tree; the sum over a multi-point leaf
+//                   equals global APSS at cutoff -> infinity (REF_CPU falls short); the
+//                   1e-6 plane threshold puts a sphere of curvature radius 6e5 on its
+//                   tangent plane under REF_GPU and on the sphere under REF_CPU.
 //   sphere fit    : noise-free sphere / plane data reproduce the exact projection.
 //   kNN / ball    : brute force against the reference kd-tree's golden vectors.
 //   adaptive octree driver : golden vectors of the reference's OctreeGenerator steps
@@ -697,6 +703,23 @@ int32_t orc_project(void* h, const orc_kernel* k, const orc_params* prm, int32_t
       out_p[3 * i] = p.x, out_p[3 * i + 1] = p.y, out_p[3 * i + 2] = p.z;
       out_n[3 * i] = n.x, out_n[3 * i + 1] = n.y, out_n[3 * i + 2] = n.z;
     }
+  }
+  return 0;
+}
+
+// The weighted sums of one MLoD (mode 0) or global (mode 1) pass for m queries, in FP64:
+// out9 per query = {sum w, sum w p (3), sum w n (3), sum w p.n, sum w |p|^2} (apssStats,
+// CPUUtils.h:51-61), before the fit.  Used to check the traversal sums term by term.
+int32_t orc_stats(void* h, const orc_kernel* k, const orc_params* prm, const float* q, uint32_t m, double* out9) {
+  auto* t = static_cast<orc::Tree*>(h);
+  if (!t || !k || !prm || (prm->mode != 0 && prm->mode != 1)) return -1;
+  for (uint32_t i = 0; i < m; ++i) {
+    const orc::V3<double> qi = orc::mk<double>(q[3 * i], q[3 * i + 1], q[3 * i + 2]);
+    const orc::Stats<double> s = prm->mode == 1 ? orc::globalStats<double>(*t, qi, *k)
+                                                : orc::approx<double>(*t, 0, qi, *k, prm->min_depth, (double)prm->cutoff,
+                                                                      prm->semantics == 1);
+    const double v[9] = {s.w, s.wp.x, s.wp.y, s.wp.z, s.wn.x, s.wn.y, s.wn.z, s.wpn, s.wpp};
+    std::memcpy(out9 + 9 * (size_t)i, v, sizeof(v));
   }
   return 0;
 }

@@ -87,3 +87,89 @@ def test_generator_errors():
         g2.generate(4)  # no kernel installed
     with pytest.raises(A.MlodError):
         A.OctreeGenerator(A.Fastapss(t, A.Kernel(type=0, sigma=0.1))).generate(13)
+
+
+def _keys(ids):
+    """node_comparator order key of (depth, x, y, z) rows: depth, then z, y, x (signed)."""
+    ids = np.asarray(ids, np.int64)
+    b = 1 << 20
+    return (ids[:, 0] << 60) | ((ids[:, 3] + b // 2) << 40) | ((ids[:, 2] + b // 2) << 20) | (ids[:, 1] + b // 2)
+
+
+def _enclosing(p, depth):
+    """aabb_octree::enclosing in [-1, 1]^3 (octree.h:956-965), float32 arithmetic."""
+    t = ((np.asarray(p, np.float32) - np.float32(-1.0)) / np.float32(2.0)) / np.float32(2.0 ** -depth)
+    return np.floor(t).astype(np.int64)
+
+
+def _centres(ids):
+    d = ids[:, :1].astype(np.float32)
+    return ((2 * ids[:, 1:] + 1).astype(np.float32) * np.exp2(-d).astype(np.float32) - np.float32(1.0)).astype(np.float32)
+
+
+@pytest.mark.timeout(600)
+def test_generate_depth10_torus1m(oracle_built):
+    """The adaptive octree of the 1M-point torus configuration (rational kernel e = 4, cutoff 3,
+    compensated form) at max depth 10, on the cloud in the reference's normalised frame
+    (Pn::normalize).  Invariants: cells unique and in node_comparator order, depths <= 10, the
+    relative depth unchanged (the cloud is inside the root cell), every point's depth-10 cell among
+    the cells; one level transition of find_stable_nodesGPU / add_children / expand
+    (upsample.cpp:274-358) checked on samples in both directions with the GPU projection; and the
+    same driver at depth 6 equal to the oracle's driver cell for cell."""
+    w = ds.make_workload("torus1m")
+    p = ds.pn_normalize(w["points"])
+    t = A.APSSOctree().build(p, w["normals"])
+    K = A.Kernel(**w["kernel"])
+    ap = A.Fastapss(t, K)
+    ap.setScalingProtectionSphere(w["cutoff"])
+    ap.set_precision("fp32c")
+    gen = A.OctreeGenerator(ap, K)
+    total = []
+    leaves = gen.generate(10, total)
+    total = total[0]
+    info = gen.info()
+    print(f"torus 1M depth 10: {len(leaves)} cells, {len(total)} crossed, {info['projected']} centres projected in "
+          f"{info['calls']} calls", flush=True)
+    assert info["max_rel_depth"] == 10 and info["nonfinite"] == 0
+    for ids in (leaves, total):
+        k = _keys(ids)
+        assert np.all(np.diff(k) > 0)  # unique, node_comparator order
+        assert ids[:, 0].max() == 10
+    assert len(leaves) > 1_000_000
+    # every point's depth-10 cell is a generated cell
+    cell = _enclosing(p, 10)
+    kl = _keys(leaves)
+    kp = _keys(np.concatenate([np.full((len(p), 1), 10), cell], 1))
+    assert np.isin(kp, kl).all()
+    # one level transition, depth 8 -> 9, on samples: S9 = expand_9(children(stable(S8)))
+    proj = _gpu_projector(ap)
+    S8 = total[total[:, 0] == 8]
+    S9k = _keys(total[total[:, 0] == 9])
+    pp, pn = proj(_centres(S8))
+    ok = np.sum(pn.astype(np.float32) ** 2, 1) >= 0.5
+    stable = np.unique(_keys(np.concatenate([np.full((int(ok.sum()), 1), 8), _enclosing(pp[ok], 8)], 1)))
+    rng = np.random.default_rng(5)
+    # (a) children of stable cells and their 26 neighbours are crossed cells at depth 9
+    sk = stable[rng.choice(len(stable), min(3000, len(stable)), replace=False)]
+    b = 1 << 19
+    sx, sy, sz = (sk & 0xFFFFF) - b, ((sk >> 20) & 0xFFFFF) - b, ((sk >> 40) & 0xFFFFF) - b
+    d = np.arange(-1, 3)
+    gx, gy, gz = np.meshgrid(d, d, d, indexing="ij")
+    off = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1)  # 4x4x4 block: children and their neighbours
+    blk = np.stack([2 * sx, 2 * sy, 2 * sz], 1)[:, None, :] + off[None]
+    bk = _keys(np.concatenate([np.full((blk.shape[0] * 64, 1), 9), blk.reshape(-1, 3)], 1))
+    assert np.isin(bk, S9k).all()
+    # (b) every sampled crossed cell at depth 9 has a stable depth-8 cell whose child is within one cell
+    xs = total[total[:, 0] == 9][rng.choice(len(S9k), 3000, replace=False)]
+    found = np.zeros(len(xs), bool)
+    for dd in off[(off[:, 0] < 2) & (off[:, 1] < 2) & (off[:, 2] < 2)]:
+        nb = xs[:, 1:] + dd
+        anc = np.floor_divide(nb, 2)
+        found |= np.isin(_keys(np.concatenate([np.full((len(xs), 1), 8), anc], 1)), stable)
+    assert found.all()
+    # the same driver on this cloud at depth 6 against the oracle's driver
+    gen6 = A.OctreeGenerator(ap, K)
+    tot6 = []
+    l6 = gen6.generate(6, tot6)
+    og = oracle_built.OracleGenerator(proj)
+    assert np.array_equal(l6, og.generate(6)) and np.array_equal(tot6[0], og.total())

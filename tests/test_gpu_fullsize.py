@@ -63,6 +63,35 @@ def test_full_size_projection_gate_and_batch_invariance(workload):
     assert np.array_equal(sp, out["fp32c"][0][idx]) and np.array_equal(sn, out["fp32c"][1][idx])
 
 
+@pytest.mark.timeout(600)
+def test_full_size_headline_vs_fp64_oracle(workload, oracle_built):
+    """The headline configuration as the bench runs it (REF_GPU semantics, the compensated FP32C
+    form) against the FP64 oracle's REF_GPU function on the same 8M-point octree (same tie
+    policy): 2000 sampled queries, 3 steps, every one within 1e-6 x diag.  The plain FP32 form on
+    the same sample is reported beside it."""
+    P, N, Q = workload["points"], workload["normals"], workload["queries"]
+    t = A.APSSOctree().build(P, N)
+    K = A.Kernel(**workload["kernel"])
+    diag = float(np.linalg.norm(P.max(0).astype(np.float64) - P.min(0)))
+    idx = np.sort(np.random.default_rng(17).choice(len(Q), 2000, replace=False))
+    got = {}
+    for prec in ("fp32c", "fp32"):
+        ap = A.Fastapss(t, K)
+        ap.setScalingProtectionSphere(workload["cutoff"])
+        ap.set_semantics("ref_gpu")
+        ap.set_precision(prec)
+        got[prec], _ = ap.project_points(Q[idx], 3)
+    o = oracle_built.OracleTree(P, N, tie_policy=0)
+    k = oracle_built.make_kernel(type=K.type, sigma=K.sigma, exponent=K.exponent, constant_shift=K.constant_shift,
+                                 sigmas=K.sigmas)
+    par = oracle_built.make_params(cutoff=workload["cutoff"], min_depth=0, semantics="ref_gpu")
+    op, _ = o.project(Q[idx], k, par, n_iter=3, precision=1, nthreads=16)
+    err = {f: np.linalg.norm(g.astype(np.float64) - op, axis=1) / diag for f, g in got.items()}
+    print(f"8M headline, 2000 queries x 3 steps vs the FP64 oracle (REF_GPU): FP32C max {err['fp32c'].max():.3e} "
+          f"RMS {np.sqrt(np.mean(err['fp32c'] ** 2)):.3e}; plain FP32 max {err['fp32'].max():.3e} x diag")
+    assert np.isfinite(err["fp32c"]).all() and err["fp32c"].max() < 1e-6
+
+
 def test_full_size_surface_mesh_is_closed(workload):
     """The surface mesh of the full cloud at depth 8 (the bench's mesh): the superquadric is closed,
     so dual contouring of its projected corner field gives a closed, consistently oriented mesh of

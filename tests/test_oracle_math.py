@@ -179,3 +179,240 @@ def test_knn_ball_brute_force(oracle_built):
              np.sum(w * np.sum(P[nb] ** 2, 1))]
         e, _ = b.oracle_fit(s, q[qi].astype(np.float64))
         assert np.allclose(op[qi], e, atol=1e-9)
+
+
+# ---------------------------------------------------------------------------------------------
+# The terms in which the reference's GPU projection (REF_GPU, Fastapss.cu) differs from its CPU
+# projection (REF_CPU, CPUUtils.cpp), which the golden vectors of the reference's CPU code pin:
+# the parent share a_child / a_node (Fastapss.cu:244; CPUUtils.cpp:485 uses 1 / #children), the
+# sum over every point of a multi-point leaf (Fastapss.cu:189-200; CPUUtils.cpp:455-456 keeps the
+# last one) and the plane threshold |u4| < 1e-6 (Fastapss.cu:362; 1e-12 at CPUUtils.cpp:751).
+
+
+def _lattice(n_axis=16):
+    """Points (2i + 1) / (2 n_axis) on a full n_axis^3 grid: every split of the octree falls
+    exactly between two grid planes, so each internal node has eight children with the same box
+    and the same area, and every leaf holds one point."""
+    g = (2 * np.arange(n_axis) + 1) / (2.0 * n_axis)
+    p = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    n = p - 0.5
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return p.astype(np.float32), n.astype(np.float32)
+
+
+def _u4(s):
+    """u4 of the algebraic sphere fitted to FP64 sums (CPUUtils.cpp:741-743)."""
+    w, sp, sn, spn, spp = s[0], s[1:4], s[4:7], s[7], s[8]
+    return 0.5 * (spn - sp @ (sn / w)) / (spp - sp @ (sp / w))
+
+
+def test_ref_gpu_share_equals_ref_cpu_on_equal_areas(oracle_built):
+    """On an octree whose siblings all have equal areas, a_child / a_node = 1 / #children, so
+    the two semantics sum the same terms: equal sums and projections (FP64).  A share written as
+    a_node / a_child, without the parent weight, or with the wrong node's area would differ."""
+    b = oracle_built
+    p, n = _lattice(16)
+    o = b.OracleTree(p, n)
+    d = o.nodes()
+    assert o.ties == 0 and np.all(d["leaf_count"][d["leaf_count"] > 0] == 1)
+    inner = d["first_child"] >= 0
+    assert np.all(d["num_children"][inner] == 8)  # a full octree of depth 4
+    a = o.moments()[1][:, 0]
+    for k in np.nonzero(inner)[0]:
+        c = d["first_child"][k] + np.arange(8)
+        assert np.all(a[c] == a[c[0]]) and a[k] == 8 * a[c[0]]
+    rng = np.random.default_rng(21)
+    q = rng.uniform(0.1, 0.9, (300, 3)).astype(np.float32)
+    k = b.make_kernel(type=2, sigmas=[0.04, 0.08, 0.16])
+    for md in (0, 2):
+        sums = {s: o.stats(q, k, b.make_params(cutoff=2.0, min_depth=md, semantics=s)) for s in ("ref_gpu", "ref_cpu")}
+        assert np.allclose(sums["ref_gpu"], sums["ref_cpu"], rtol=1e-12, atol=1e-300)
+        # both fits away from the plane thresholds, so the same branch in both semantics
+        u4 = np.array([_u4(s) for s in sums["ref_gpu"]])
+        assert np.all(np.abs(u4) > 1e-3)
+        pr = {s: o.project(q, k, b.make_params(cutoff=2.0, min_depth=md, semantics=s), n_iter=2, precision=1)
+              for s in ("ref_gpu", "ref_cpu")}
+        assert np.abs(pr["ref_gpu"][0] - pr["ref_cpu"][0]).max() < 1e-12
+        assert np.abs(pr["ref_gpu"][1] - pr["ref_cpu"][1]).max() < 1e-12
+
+
+def test_ref_gpu_sums_every_point_of_a_leaf(oracle_built):
+    """Leaves of several points, where the semantics differ.  With a cutoff beyond every node no
+    node is far and every blend factor is 1, so REF_GPU, which sums every point of a leaf, equals
+    global APSS over all points; REF_CPU, which keeps a leaf's last point, falls short by the other
+    points' terms.  Two trees: duplicated points, which stay together down to depth 12 (small
+    leaves, so a small shortfall), and the octree cut at depth 4 (leaves of up to 9 points)."""
+    b = oracle_built
+    p, n = ds.sphere(1500, seed=14, noise=0.01)  # noisy: every subset fits a different sphere
+    p = np.concatenate([p, p[:40], p[:10]])
+    n = np.concatenate([n, n[:40], n[:10]])
+    R = ds.bbox_radius(p)
+    k = b.make_kernel(type=2, sigmas=[0.05 * R, 0.1 * R])
+    q = (p[:120] + 0.01).astype(np.float32)  # includes queries next to the duplicates
+    for max_depth, shortfall in ((12, 1e-10), (4, 1e-2)):
+        o = b.OracleTree(p, n, max_depth=max_depth)
+        d = o.nodes()
+        multi = d["leaf_count"] > 1
+        if max_depth == 12:
+            assert multi.sum() == 40 and np.all(d["depth"][multi] == 12)
+        else:
+            assert np.all(d["depth"][d["leaf_count"] > 0] <= 4) and d["leaf_count"].max() >= 8
+        g = o.stats(q, k, b.make_params(mode="global", semantics="ref_gpu"))
+        mg = o.stats(q, k, b.make_params(cutoff=1e6, semantics="ref_gpu"))
+        mc = o.stats(q, k, b.make_params(cutoff=1e6, semantics="ref_cpu"))
+        assert np.allclose(mg, g, rtol=1e-12, atol=0)
+        assert np.all(mc[:, 0] < g[:, 0] * (1 - shortfall))
+        gp, _ = o.project(q, k, b.make_params(mode="global", semantics="ref_gpu"), precision=1)
+        mp, _ = o.project(q, k, b.make_params(cutoff=1e6, semantics="ref_gpu"), precision=1)
+        assert np.abs(mp - gp).max() < 1e-12
+        if max_depth == 4:
+            cp, _ = o.project(q, k, b.make_params(cutoff=1e6, semantics="ref_cpu"), precision=1)
+            assert np.abs(cp - gp).max(1).min() > 1e-6
+
+
+def _big_sphere_patch(npts, R, seed):
+    """Samples of the sphere of radius R centred at (0, 0, -R) within distance 1 of the axis,
+    with exact radial normals: the surface near the origin, curvature 1 / R."""
+    rng = np.random.default_rng(seed)
+    rr = np.sqrt(rng.uniform(0, 1, npts))
+    ph = rng.uniform(0, 2 * np.pi, npts)
+    x, y = rr * np.cos(ph), rr * np.sin(ph)
+    z = np.sqrt(R * R - x * x - y * y) - R
+    p = np.stack([x, y, z], 1)
+    nrm = (p - np.array([0.0, 0.0, -R])) / R
+    return p, nrm
+
+
+def test_plane_threshold_per_semantics_fit(oracle_built):
+    """Sums of exact samples of a sphere of radius R = 6e5 give u4 = 1 / (2R) = 8.3e-7, between
+    the two thresholds: the REF_GPU fit (|u4| < 1e-6) projects onto the plane u0 + u.x = 0, here
+    z = 0, with the normal of the summed normals; the REF_CPU fit (|u4| < 1e-12) onto the sphere,
+    c + R (q - c) / |q - c|.  The two answers are 2e-7 - 7e-7 apart at these queries."""
+    b = oracle_built
+    R = 6.0e5
+    c = np.array([0.0, 0.0, -R])
+    p, nrm = _big_sphere_patch(2000, R, seed=31)
+    rng = np.random.default_rng(32)
+    w = rng.uniform(0.2, 1.5, len(p))
+    s = np.concatenate([[w.sum()], w @ p, w @ nrm, [w @ (p * nrm).sum(1)], [w @ (p * p).sum(1)]])
+    assert abs(_u4(s) - 1 / (2 * R)) < 1e-3 / (2 * R)
+    for _ in range(20):
+        rho, ph = rng.uniform(0.5, 0.9), rng.uniform(0, 2 * np.pi)
+        q = np.array([rho * np.cos(ph), rho * np.sin(ph), rng.uniform(-0.01, 0.01)])
+        sphere_p = c + R * (q - c) / np.linalg.norm(q - c)
+        plane_p = np.array([q[0], q[1], 0.0])
+        gp, gn = b.oracle_fit(s, q, ref_gpu=True)
+        cp, cn = b.oracle_fit(s, q, ref_gpu=False)
+        assert np.abs(gp - plane_p).max() < 1e-8 and np.abs(gp - sphere_p).max() > 1e-7
+        assert np.abs(cp - sphere_p).max() < 1e-8 and np.abs(cp - plane_p).max() > 1e-7
+        assert np.allclose(gn, s[4:7] / np.linalg.norm(s[4:7]), atol=1e-12)
+        assert np.allclose(cn, (sphere_p - c) / R, atol=1e-9)
+
+
+def test_plane_threshold_per_semantics_tree(oracle_built):
+    """The same through the MLoD projection of a float32 cloud on that sphere: the MLoD sums are a
+    positive combination of exact sphere samples, so REF_GPU lands on the plane z = 0 and REF_CPU
+    on the sphere, each within rounding."""
+    b = oracle_built
+    R = 6.0e5
+    c = np.array([0.0, 0.0, -R])
+    p, nrm = _big_sphere_patch(4000, R, seed=33)
+    o = b.OracleTree(p.astype(np.float32), nrm.astype(np.float32))
+    rng = np.random.default_rng(34)
+    rho, ph = rng.uniform(0.5, 0.9, 40), rng.uniform(0, 2 * np.pi, 40)
+    q = np.stack([rho * np.cos(ph), rho * np.sin(ph), rng.uniform(-0.01, 0.01, 40)], 1).astype(np.float32)
+    k = b.make_kernel(type=2, sigmas=[0.3, 0.6])
+    par = {s: b.make_params(cutoff=2.0, semantics=s) for s in ("ref_gpu", "ref_cpu")}
+    u4 = np.array([_u4(s) for s in o.stats(q, k, par["ref_gpu"])])
+    assert np.all((u4 > 1e-12) & (u4 < 1e-6))
+    gp, _ = o.project(q, k, par["ref_gpu"], precision=1, step_max=1.0)
+    cp, _ = o.project(q, k, par["ref_cpu"], precision=1, step_max=1.0)
+    q64 = q.astype(np.float64)
+    sphere_p = c + R * (q64 - c) / np.linalg.norm(q64 - c, axis=1, keepdims=True)
+    plane_p = np.concatenate([q64[:, :2], np.zeros((len(q), 1))], 1)
+    assert np.abs(gp - plane_p).max() < 1e-8 and np.abs(gp - sphere_p).max(1).min() > 1e-7
+    assert np.abs(cp - sphere_p).max() < 1e-8 and np.abs(cp - plane_p).max(1).min() > 1e-7
+
+
+def test_two_level_tree_sums_by_hand(oracle_built):
+    """Three points A = 0, C = (5/8)(1, 1, 1), B = (1, 1, 1): the root box [0, 1]^3 splits at 1/2
+    into leaf A (box [0, 1/2]^3) and node X (box [1/2, 1]^3), which splits at 3/4 into leaves C and
+    B.  The MLoD sums (CPUUtils.cpp:478-501, Fastapss.cu:238-299) written out for this tree:
+        root near: sum over c in (A, X) of  b_c value(c) + (1 - b_c) share_c w(pbar_root) M_root
+        X near:    value(X) = sum over c in (C, B) of  b'_c w(p_c) M_c + (1 - b'_c) share'_c w(pbar_X) M_X
+        X far:     value(X) = w(pbar_X) M_X
+    with share_c = a_c / a_parent under REF_GPU (here 2/3, 1/3 at the root) and 1 / #children under
+    REF_CPU, b_c = (1 - cos(3.14159265 t)) / 2, t = dP / (dP + dC)."""
+    b = oracle_built
+    P = np.array([[0, 0, 0], [1, 1, 1], [0.625, 0.625, 0.625]], np.float32)  # A, B, C
+    N = np.array([[-1, -1, -1], [1, 1, 1], [1, 1, 2]], np.float64)
+    N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(np.float32)
+    o = b.OracleTree(P, N)
+    d = o.nodes()
+    # BFS: root, A (slot 0), X (slot 7), C (slot 0 of X), B (slot 7 of X)
+    assert list(d["depth"]) == [0, 1, 1, 2, 2] and list(d["num_children"]) == [2, 0, 2, 0, 0]
+    assert list(d["leaf_idx"]) == [0, 2, 1]
+    area = o.areas().astype(np.float64)  # (leaf diagonal)^2 in float: ~0.75 (A), ~0.1875 (B, C)
+    assert abs(area[0] - 0.75) < 1e-6 and abs(area[1] - 0.1875) < 1e-6 and area[1] == area[2]
+
+    def mom(i):
+        a, p, nn = area[i], P[i].astype(np.float64), N[i].astype(np.float64)
+        return np.concatenate([[a], a * p, a * nn, [a * p @ nn], [a * p @ p]])
+
+    MA, MB, MC = mom(0), mom(1), mom(2)
+    MX = MC + MB
+    Mr = MA + MX
+
+    def rad(w):  # BBOX::radius of a cube of side w: sqrt(3 w^2) / 2 in float
+        return float(np.float32(np.sqrt(np.float32(np.float32(w * w) * np.float32(3.0)))) / np.float32(2.0))
+
+    c_root, r_root = np.full(3, 0.5), rad(1.0)
+    c_A, c_X, r_half = np.full(3, 0.25), np.full(3, 0.75), rad(0.5)
+    c_C, c_B, r_q = np.full(3, 0.625), np.full(3, 0.875), rad(0.25)
+    sig = np.float32([0.3, 0.6]).astype(np.float64)
+    k = b.make_kernel(type=2, sigmas=[0.3, 0.6])
+    cut = 1.0  # small enough that children lie outside their own cutoff sphere: 0 < b_c < 1
+
+    def w(M, q):
+        pb = M[1:4] / M[0]
+        d2 = np.sum((q - pb) ** 2)
+        return float(np.sum(np.exp(-d2 / (2 * sig * sig)) / sig ** 3))
+
+    def beta(q, cp, rp, cc, rc):
+        dP = abs(np.linalg.norm(q - cp) - cut * rp)
+        dC = max(np.linalg.norm(q - cc) - cut * rc, 0.0)
+        return (1 - np.cos(3.14159265 * dP / (dP + dC))) / 2
+
+    def expect(q, gpu):
+        assert np.linalg.norm(q - c_root) <= cut * r_root  # the root is near
+        wr = w(Mr, q)
+        out = np.zeros(9)
+        bA = beta(q, c_root, r_root, c_A, r_half)
+        bX = beta(q, c_root, r_root, c_X, r_half)
+        sA, sX = (MA[0] / Mr[0], MX[0] / Mr[0]) if gpu else (0.5, 0.5)
+        out += bA * w(MA, q) * MA + (1 - bA) * sA * wr * Mr
+        if np.linalg.norm(q - c_X) > cut * r_half:
+            vX = w(MX, q) * MX
+        else:
+            wx = w(MX, q)
+            bC = beta(q, c_X, r_half, c_C, r_q)
+            bB = beta(q, c_X, r_half, c_B, r_q)
+            sC, sB = (MC[0] / MX[0], MB[0] / MX[0]) if gpu else (0.5, 0.5)
+            vX = bC * w(MC, q) * MC + (1 - bC) * sC * wx * MX + bB * w(MB, q) * MB + (1 - bB) * sB * wx * MX
+        return out + bX * vX + (1 - bX) * sX * wr * Mr
+
+    # root near for all; X near for the 1st, 4th, 6th, 7th and far for the others; blend factors
+    # strictly between 0 and 1 at the root (A or X) and inside X (C or B)
+    Q = np.array([[0.6, 0.7, 0.8], [0.2, 0.1, 0.3], [0.55, 0.4, 0.5], [0.9, 0.95, 0.7], [0.3, 0.3, 0.3],
+                  [0.7, 0.7, 0.7], [0.5, 0.6, 0.55]], np.float32)
+    q0 = Q[0].astype(np.float64)
+    assert 0.5 < beta(q0, c_root, r_root, c_A, r_half) < 0.7 and 0.7 < beta(q0, c_X, r_half, c_B, r_q) < 0.9
+    for sem, gpu in (("ref_gpu", True), ("ref_cpu", False)):
+        got = o.stats(Q, k, b.make_params(cutoff=cut, semantics=sem))
+        for qi, g in zip(Q, got):
+            e = expect(qi.astype(np.float64), gpu)
+            assert np.allclose(g, e, rtol=1e-12, atol=1e-14 * np.abs(e).max()), (sem, qi, g, e)
+    # the shares of the two semantics differ here, and so do the sums
+    e_g = expect(Q[0].astype(np.float64), True)
+    e_c = expect(Q[0].astype(np.float64), False)
+    assert np.abs(e_g - e_c).max() > 1e-3 * np.abs(e_g).max()
