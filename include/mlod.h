@@ -148,8 +148,9 @@ int mlod_tree_dump(const mlod_tree* t, int32_t* depth, float* bb6, float* center
 
 /* Traversal counters of one step over the uploaded queries (untimed).  out6 receives: far
  * nodes accepted, leaf points summed, child nodes tested, near nodes adding their own share,
- * traversal loop iterations summed over queries, and 64 x the per-wavefront maximum of those
- * iterations summed over wavefronts (so out6[4] / out6[5] is the SIMD lane efficiency). */
+ * traversal loop iterations summed over queries, and the lane slots those iterations were
+ * issued in (64 x the wavefronts' traversal loop iterations), so out6[4] / out6[5] is the
+ * SIMD lane efficiency. */
 int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out6, void* stream);
 
 /* ---------------------------------------------------------------------------------------

@@ -374,7 +374,7 @@ class APSS:
 
     def count_visits(self, n=None, stream=None):
         """Traversal counters of one step: far nodes, leaf points, children tested, near nodes'
-        own shares, loop iterations, and 64 x per-wavefront maximum iterations."""
+        own shares, lane loop iterations, and the lane slots they were issued in (64 x wavefront iterations)."""
         n = self._m if n is None else int(n)
         self._sync_params()
         out = (C.c_uint64 * 6)()

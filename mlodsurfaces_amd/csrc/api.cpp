@@ -265,9 +265,8 @@ int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out6, void* stream) {
   if (n > t->qs.m) return fail(MLOD_ERR_INVALID, "n exceeds the number of uploaded queries");
   hipStream_t s = (hipStream_t)stream;
   const uint32_t m = t->qs.m;
-  const size_t waves = ((size_t)m + 63) / 64;
-  // 5 totals, then one 32-bit per-wavefront maximum of loop iterations per wavefront
-  const size_t cbytes = 5 * sizeof(unsigned long long) + waves * sizeof(uint32_t);
+  // far, leaf points, tested, near, lane iterations, wavefront iterations (the longest lane's)
+  const size_t cbytes = 6 * sizeof(unsigned long long);
   float4 *bq = nullptr, *bn = nullptr;
   unsigned long long* dc = nullptr;
   if (hipMalloc((void**)&bq, (size_t)m * sizeof(float4) + 16) != hipSuccess ||
@@ -280,20 +279,16 @@ int mlod_count_visits(mlod_tree* t, uint32_t n, uint64_t* out6, void* stream) {
   if (e == hipSuccess) e = hipMemcpyAsync(bn, t->qs.qn, (size_t)m * sizeof(float4), hipMemcpyDeviceToDevice, s);
   if (e == hipSuccess) e = hipMemsetAsync(dc, 0, cbytes, s);
   int r = e == hipSuccess ? launch_count(t, n, dc, s) : hip_fail(e, "counter pass setup", __FILE__, __LINE__);
-  unsigned long long h[5] = {0, 0, 0, 0, 0};
-  std::vector<uint32_t> wmax(waves);
+  unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
   if (r == MLOD_OK) {
     e = hipMemcpyAsync(h, dc, sizeof(h), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(wmax.data(), dc + 5, waves * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(t->qs.q, bq, (size_t)m * sizeof(float4), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(t->qs.qn, bn, (size_t)m * sizeof(float4), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) r = hip_fail(e, "counter pass", __FILE__, __LINE__);
   }
   (void)hipFree(bq), (void)hipFree(bn), (void)hipFree(dc);
-  uint64_t laneSlots = 0;
-  for (uint32_t w : wmax) laneSlots += 64ull * w;
+  const uint64_t laneSlots = 64ull * h[5];
   for (int k = 0; k < 5; ++k) out6[k] = h[k];
   out6[5] = laneSlots;
   return r;

@@ -34,9 +34,8 @@ constexpr int PBS = 128;         // lanes per workgroup
 // Levels of the LDS stack of waiting nodes.  A node waits only while it has more than one near
 // child still to expand, and near children are internal nodes; internal nodes have depth
 // 0..kMaxDepth - 1 and their children depth <= kMaxDepth, so a waiting node has depth at most
-// kMaxDepth - 2 (its near children are internal, hence of depth <= kMaxDepth - 1).  One level
-// fewer than the depth count keeps the per-workgroup LDS at 11 x 9 x 128 B, which lets one more
-// workgroup per CU reside (six waves per SIMD, as many as the kernel's registers allow).
+// kMaxDepth - 2 (its near children are internal, hence of depth <= kMaxDepth - 1): 11 levels,
+// 13 B each per lane in the FP32 forms (18.3 KB per workgroup of 128 lanes).
 constexpr int kLevels = kMaxDepth - 1;
 
 #define LAUNCH_CHECK()                                                           \
@@ -156,11 +155,21 @@ struct AccP32 {
   float w, px, py, pz, nx, ny, nz, pn, pp;
   __device__ __forceinline__ void init() { w = px = py = pz = nx = ny = nz = pn = pp = 0.f; }
 };
-// FORM 1: compensated, query-frame
+// FORM 1: compensated, query-frame.  Terms go by one FMA each into plain FP32 partial sums (the
+// l* members), which are added into the Kahan-compensated sums every kFlush traversal iterations
+// and at the end of a walk (flush()): the compensation is paid once per block of terms, and a
+// block's rounding error is bounded by its few terms.
+constexpr uint32_t kFlush = 8;
 struct AccC32 {
   KSum w, px, py, pz, nx, ny, nz, pn, pp;
+  float lw, lpx, lpy, lpz, lnx, lny, lnz, lpn, lpp;
   __device__ __forceinline__ void init() {
     w.init(), px.init(), py.init(), pz.init(), nx.init(), ny.init(), nz.init(), pn.init(), pp.init();
+    lw = lpx = lpy = lpz = lnx = lny = lnz = lpn = lpp = 0.f;
+  }
+  __device__ __forceinline__ void flush() {
+    w.add(lw), px.add(lpx), py.add(lpy), pz.add(lpz), nx.add(lnx), ny.add(lny), nz.add(lnz), pn.add(lpn), pp.add(lpp);
+    lw = lpx = lpy = lpz = lnx = lny = lnz = lpn = lpp = 0.f;
   }
 };
 // FORM 2: plain FP64
@@ -202,15 +211,15 @@ __device__ __forceinline__ void add_point(typename FormT<FORM>::Acc& acc, const 
     const float dx = P.x - q.x, dy = P.y - q.y, dz = P.z - q.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
     const float fa = f * weight32<KT>(kp, d2) * P.w;
-    acc.w.add(fa);
-    acc.px.add(fa * dx);
-    acc.py.add(fa * dy);
-    acc.pz.add(fa * dz);
-    acc.nx.add(fa * N.x);
-    acc.ny.add(fa * N.y);
-    acc.nz.add(fa * N.z);
-    acc.pn.add(fa * (dx * N.x + dy * N.y + dz * N.z));
-    acc.pp.add(fa * d2);
+    acc.lw += fa;
+    acc.lpx = fmaf(fa, dx, acc.lpx);
+    acc.lpy = fmaf(fa, dy, acc.lpy);
+    acc.lpz = fmaf(fa, dz, acc.lpz);
+    acc.lnx = fmaf(fa, N.x, acc.lnx);
+    acc.lny = fmaf(fa, N.y, acc.lny);
+    acc.lnz = fmaf(fa, N.z, acc.lnz);
+    acc.lpn = fmaf(fa, dx * N.x + dy * N.y + dz * N.z, acc.lpn);
+    acc.lpp = fmaf(fa, d2, acc.lpp);
   } else {
     const double a = P.w, x = P.x, y = P.y, z = P.z;
     const double apx = a * x, apy = a * y, apz = a * z;
@@ -278,15 +287,15 @@ __device__ __forceinline__ void add_mom(typename FormT<FORM>::Acc& acc, const KP
     const float d2 = dx * dx + dy * dy + dz * dz;
     const float fw = f * weight32<KT>(kp, d2);
     const float fa = fw * Af;
-    acc.w.add(fa);
-    acc.px.add(fa * dx);
-    acc.py.add(fa * dy);
-    acc.pz.add(fa * dz);
-    acc.nx.add(fw * m1.x);
-    acc.ny.add(fw * m1.y);
-    acc.nz.add(fw * m1.z);
-    acc.pn.add(fw * (m0.w + (dx * m1.x + dy * m1.y + dz * m1.z)));
-    acc.pp.add(fw * (m1.w + Af * d2));
+    acc.lw += fa;
+    acc.lpx = fmaf(fa, dx, acc.lpx);
+    acc.lpy = fmaf(fa, dy, acc.lpy);
+    acc.lpz = fmaf(fa, dz, acc.lpz);
+    acc.lnx = fmaf(fw, m1.x, acc.lnx);
+    acc.lny = fmaf(fw, m1.y, acc.lny);
+    acc.lnz = fmaf(fw, m1.z, acc.lnz);
+    acc.lpn = fmaf(fw, m0.w + (dx * m1.x + dy * m1.y + dz * m1.z), acc.lpn);
+    acc.lpp = fmaf(fw, m1.w + Af * d2, acc.lpp);
   } else {
     const double4 m0 = mm.m0, m1 = mm.m1;
     const double A = mm.A;
@@ -351,11 +360,14 @@ __device__ __forceinline__ T tratio(T a, T b) {  // a / b
     return a / b;
 }
 
+// A node whose near children wait, per level and lane: what its next near child needs to start
+// (the node's first child, multiplier and distance term), so a lane resumes it without reloading it.
 template <class T>
 struct LevelState {
-  uint32_t (*node)[PBS];  // node whose near children wait
-  uint8_t (*mask)[PBS];   // its near children still to expand
-  T (*mul)[PBS];          // its multiplier
+  uint32_t (*first)[PBS];  // the node's first child
+  uint8_t (*mask)[PBS];    // its near children still to expand
+  T (*mul)[PBS];           // its multiplier
+  T (*dist)[PBS];          // its distance term |d - cutoff r| (negative: no blending below it)
 };
 
 // Flattened traversal.  The lane's current node (expanded: near, or above minDepth) has its
@@ -367,45 +379,59 @@ struct LevelState {
 // per iteration whatever their position in their own tree walk, instead of waiting for the
 // widest child loop of the wavefront at every level.
 //
-// Each iteration issues all its global loads at once (X's geometry and header, and the moments
-// of the node it may contribute: X itself when testing, the finished node when moving on), so
-// one memory round trip covers the iteration.  A finished node with more than one near child
-// leaves {node, remaining near children, multiplier} in LDS for its depth (9 B per level and
-// lane, a per-lane bit set marks the levels in use); when a lane's path runs out, the deepest
-// such node is reloaded and becomes the current node again with its remaining near children,
-// its distance term recomputed from its geometry (the same operations, so the same bits).
+// Each iteration loads X's geometry and header, and the moments of the node it contributes (X
+// itself when testing, the finished node when moving on).  A finished node with more than one
+// near child leaves {first child, remaining near children, multiplier, distance term} in LDS for
+// its depth (13 B per level and lane, a per-lane bit set marks the levels in use); when a lane's
+// path runs out, the deepest such node's next near child is taken from there directly.
+//
+// The walk is resumable: begin() sets it up for one query step, iter() advances it by one
+// iteration, and `active` drops when the step's sum is complete, so a kernel can interleave
+// the walks of successive queries on one lane.
 template <int FORM, int SEM, int KT, bool CNT, class KP>
-__device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv,
-                         V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff, uint32_t minDepth,
-                         LevelState<typename FormT<FORM>::T> st, int tid, Counters& ct) {
+struct Walker {
   using T = typename FormT<FORM>::T;
-  const uint4 h0 = tv.hdr[0];
-  if (hdr_leaf(h0.y)) {
-    add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, h0, q, T(1), ct);
-    return;
+  using Acc = typename FormT<FORM>::Acc;
+  uint32_t node, first, nc, k, mask, pend;
+  int lvl;
+  bool blendNode, active;
+  T m, distP, shareSum, shareScale;
+  float Ai;
+
+  __device__ __forceinline__ void begin(Acc& acc, const KP& kp, const TreeView& tv, V3<T> q, T cutoff,
+                                        uint32_t minDepth, Counters& ct) {
+    active = false;
+    const uint4 h0 = tv.hdr[0];
+    if (hdr_leaf(h0.y)) {
+      add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, h0, q, T(1), ct);
+      return;
+    }
+    const float4 g0 = tv.geo[0];
+    const T d0 = tdist<FORM>(q, g0);
+    if (minDepth == 0 && d0 > (T)g0.w * cutoff) {
+      add_node<FORM, KT>(acc, kp, tv, 0, __uint_as_float(h0.z), q, T(1));
+      if constexpr (CNT) ct.far_ += 1;
+      return;
+    }
+    // current node = root
+    node = 0, first = h0.x, nc = hdr_count(h0.y), k = 0, mask = 0, pend = 0;
+    lvl = 0;
+    blendNode = minDepth == 0;
+    m = T(1);
+    distP = blendNode ? fabs(d0 - (T)g0.w * cutoff) : T(-1);
+    shareSum = T(0);
+    Ai = __uint_as_float(h0.z);
+    shareScale = SEM == MLOD_SEM_REF_CPU ? tratio<FORM>(T(1), (T)nc) : tratio<FORM>(T(1), (T)Ai);
+    active = true;
   }
-  const float4 g0 = tv.geo[0];
-  const T d0 = tdist<FORM>(q, g0);
-  if (minDepth == 0 && d0 > (T)g0.w * cutoff) {
-    add_node<FORM, KT>(acc, kp, tv, 0, __uint_as_float(h0.z), q, T(1));
-    if constexpr (CNT) ct.far_ += 1;
-    return;
-  }
-  // current node = root
-  uint32_t node = 0, first = h0.x, nc = hdr_count(h0.y), k = 0, mask = 0, pend = 0;
-  int lvl = 0;
-  bool blendNode = minDepth == 0;
-  T m = T(1), distP = blendNode ? fabs(d0 - (T)g0.w * cutoff) : T(-1), shareSum = T(0);
-  float Ai = __uint_as_float(h0.z);
-  T shareScale = SEM == MLOD_SEM_REF_CPU ? tratio<FORM>(T(1), (T)nc) : tratio<FORM>(T(1), (T)Ai);
-  bool active = true;
-  while (active) {
+
+  __device__ __forceinline__ void iter(Acc& acc, const KP& kp, const TreeView& tv, V3<T> q, T cutoff,
+                                       uint32_t minDepth, LevelState<T> st, int tid, Counters& ct) {
     if constexpr (CNT) ct.iters += 1;
-    // --- pick X: the next child of the current node, the next near node to expand, or a
-    // node with near children still waiting, to restore
+    // --- pick X: the next child of the current node, or the next near node to expand (of the
+    // finished node, or of the deepest node whose near children wait)
     const bool testing = k < nc;
-    bool restore = false;
-    uint32_t x = 0, rmask = 0;
+    uint32_t x = 0;
     T mp = T(0), dp = T(-1);
     bool ownContrib = false;
     const uint32_t ownNode = node;
@@ -422,29 +448,35 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
       if (mask) {
         // the finished node's first near child is next; the rest wait at this level
         const uint32_t rest = mask & (mask - 1u);
+        mp = m;
+        dp = blendNode ? distP : T(-1);
         if (rest) {
-          st.node[lvl][tid] = node;
+          st.first[lvl][tid] = first;
           st.mask[lvl][tid] = (uint8_t)rest;
-          st.mul[lvl][tid] = m;
+          st.mul[lvl][tid] = mp;
+          st.dist[lvl][tid] = dp;
           pend |= 1u << lvl;
         }
         x = first + (uint32_t)__builtin_ctz(mask);
-        mp = m;
-        dp = blendNode ? distP : T(-1);
         lvl = lvl + 1;
       } else if (pend) {
+        // the deepest waiting node's next near child
         const int L = 31 - __builtin_clz(pend);
-        pend &= ~(1u << L);
-        x = st.node[L][tid];
-        rmask = st.mask[L][tid];
+        const uint32_t wm = st.mask[L][tid];
+        const uint32_t rest = wm & (wm - 1u);
+        x = st.first[L][tid] + (uint32_t)__builtin_ctz(wm);
         mp = st.mul[L][tid];
-        restore = true;
-        lvl = L;
+        dp = st.dist[L][tid];
+        if (rest)
+          st.mask[L][tid] = (uint8_t)rest;
+        else
+          pend &= ~(1u << L);
+        lvl = L + 1;
       } else {
         active = false;
       }
     }
-    // --- one round of loads: X's geometry and header, and the moments that may be added
+    // --- loads: X's geometry and header, and the moments that may be added
     const uint32_t tgt = testing ? x : ownNode;
     const NodeMom<FORM> mm = load_mom<FORM>(tv, tgt);
     float4 gx = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -465,7 +497,7 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
         const T dc = fmax(dxq - rc, T(0));
         beta = blend(tratio<FORM>(dp, dp + dc));
       }
-      const T mx = restore ? mp : mp * beta;
+      const T mx = mp * beta;
       const float Ax = __uint_as_float(hx.z);
       if (testing) {
         ++k;
@@ -478,7 +510,7 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
         const bool leaf = hdr_leaf(hx.y);
         const bool single = leaf && hdr_count(hx.y) == 1u;
         const bool far = !leaf && hdr_depth(hx.y) >= minDepth && dxq > rc;
-        if (leaf && !single) add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, hx, q, mx, ct);  // depth-12 multi-point leaf
+        if (leaf && !single) add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, hx, q, mx, ct);  // multi-point leaf
         if (!leaf && !far) mask |= 1u << (k - 1u);
         if constexpr (CNT) {
           ct.far_ += far ? 1u : 0u;
@@ -488,13 +520,12 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
         At = Ax;
         f = mx;
       } else {
-        // X becomes the current node: a new one (children to test) or a restored one
-        // (children tested, near children rmask still to expand, own share already added)
+        // X becomes the current node, its children to test
         node = x;
         first = hx.x;
         nc = hdr_count(hx.y);
-        k = restore ? nc : 0u;
-        mask = rmask;
+        k = 0u;
+        mask = 0u;
         blendNode = hdr_depth(hx.y) >= minDepth;
         distP = fabs(dxq - rc);
         m = mx;
@@ -505,6 +536,29 @@ __device__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const Tre
     }
     if (contrib) add_mom<FORM, KT>(acc, kp, mm, At, q, f);
   }
+};
+
+// The whole walk of one query step on one lane.
+template <int FORM, int SEM, int KT, bool CNT, class KP>
+__device__ __forceinline__ void traverse(typename FormT<FORM>::Acc& acc, const KP& kp, const TreeView& tv,
+                                         V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff,
+                                         uint32_t minDepth, LevelState<typename FormT<FORM>::T> st, int tid,
+                                         Counters& ct) {
+  Walker<FORM, SEM, KT, CNT, KP> w;
+  w.begin(acc, kp, tv, q, cutoff, minDepth, ct);
+  // the lanes of a wavefront start together and advance one iteration per loop, so the
+  // compensated form's block flushes fall on the same loop iterations for all of them
+  uint32_t itc = 0;
+  while (w.active) {
+    w.iter(acc, kp, tv, q, cutoff, minDepth, st, tid, ct);
+    if constexpr (FORM == 1) {
+      if (++itc == kFlush) {
+        acc.flush();
+        itc = 0;
+      }
+    }
+  }
+  if constexpr (FORM == 1) acc.flush();
 }
 
 // ----------------------------------------------------------------------------- sphere fit
@@ -615,47 +669,55 @@ __device__ __forceinline__ void fit_step(const typename FormT<FORM>::Acc& acc, V
 
 // ----------------------------------------------------------------------------- kernels
 // MLoD projection of the sorted queries q[0..m) in place (projectCuda, Fastapss.cu:398-437):
-// n_iter steps per query; lanes whose caller index perm[i] >= limit are skipped.
+// n_iter steps per query, one lane per query; queries whose caller index perm[i] >= limit are
+// skipped.  A wavefront's 64 lanes hold 64 consecutive (Morton-ordered) queries, so they start
+// their walks together and load the same nodes at the same iterations for much of them.
 template <int FORM, int SEM, int KT, bool CNT>
 __global__ __launch_bounds__(PBS) void k_project(TreeView tv, KParams kp, float4* __restrict__ q, float4* __restrict__ qn,
                                                  const uint32_t* __restrict__ perm, uint32_t m, uint32_t limit,
                                                  uint32_t nIter, float stepMax, float cutoff, uint32_t minDepth,
                                                  unsigned long long* __restrict__ counters) {
   using T = typename FormT<FORM>::T;
-  __shared__ uint32_t sNode[kLevels][PBS];
+  __shared__ uint32_t sFirst[kLevels][PBS];
   __shared__ uint8_t sMask[kLevels][PBS];
   __shared__ T sMul[kLevels][PBS];
+  __shared__ T sDist[kLevels][PBS];
   const int tid = threadIdx.x;
   const uint32_t i = xcd_block() * PBS + tid;
-  if (i >= m) return;
-  if (limit < m && perm[i] >= limit) return;
-  const LevelState<T> st{sNode, sMask, sMul};
-  const float4 q4 = q[i];
-  V3<double> qd{q4.x, q4.y, q4.z};
-  float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
+  const bool live = i < m && !(limit < m && perm[i] >= limit);
   Counters ct{0, 0, 0, 0, 0};
-  for (uint32_t it = 0; it < nIter; ++it) {
-    typename FormT<FORM>::Acc acc;
-    acc.init();
-    const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
-    V3<T> qt;
-    if constexpr (FORM == 2)
-      qt = V3<T>{qd.x, qd.y, qd.z};
-    else
-      qt = V3<T>{qf.x, qf.y, qf.z};
-    traverse<FORM, SEM, KT, CNT>(acc, kp, tv, qt, (T)cutoff, minDepth, st, tid, ct);
-    fit_step<FORM, SEM>(acc, qd, qf, stepMax, nout);
+  if (live) {
+    const LevelState<T> st{sFirst, sMask, sMul, sDist};
+    const float4 q4 = q[i];
+    V3<double> qd{q4.x, q4.y, q4.z};
+    float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t it = 0; it < nIter; ++it) {
+      typename FormT<FORM>::Acc acc;
+      acc.init();
+      const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
+      V3<T> qt;
+      if constexpr (FORM == 2)
+        qt = V3<T>{qd.x, qd.y, qd.z};
+      else
+        qt = V3<T>{qf.x, qf.y, qf.z};
+      traverse<FORM, SEM, KT, CNT>(acc, kp, tv, qt, (T)cutoff, minDepth, st, tid, ct);
+      fit_step<FORM, SEM>(acc, qd, qf, stepMax, nout);
+    }
+    q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
+    qn[i] = nout;
   }
-  q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
-  qn[i] = nout;
   if constexpr (CNT) {
-    atomicAdd(&counters[0], (unsigned long long)ct.far_);
-    atomicAdd(&counters[1], (unsigned long long)ct.leafPts);
-    atomicAdd(&counters[2], (unsigned long long)ct.tested);
-    atomicAdd(&counters[3], (unsigned long long)ct.nearN);
-    atomicAdd(&counters[4], (unsigned long long)ct.iters);
-    // per-wavefront maximum of the lanes' loop iterations, one slot per wavefront after the totals
-    atomicMax((unsigned int*)(counters + 5) + (i >> 6), ct.iters);
+    // lane slots: 64 x the longest loop of the wavefront (lanes wait for it)
+    uint32_t wmax = ct.iters;
+    for (int o = 32; o > 0; o >>= 1) wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, o));
+    if (live) {
+      atomicAdd(&counters[0], (unsigned long long)ct.far_);
+      atomicAdd(&counters[1], (unsigned long long)ct.leafPts);
+      atomicAdd(&counters[2], (unsigned long long)ct.tested);
+      atomicAdd(&counters[3], (unsigned long long)ct.nearN);
+      atomicAdd(&counters[4], (unsigned long long)ct.iters);
+    }
+    if (__lane_id() == 0) atomicAdd(&counters[5], (unsigned long long)wmax);
   }
 }
 
@@ -695,7 +757,13 @@ __global__ __launch_bounds__(PBS) void k_global(TreeView tv, KParams kp, float4*
         TreeView lv = tv;
         lv.pts = sP;
         lv.nrm = sN;
-        for (uint32_t j = 0; j < cnt; ++j) add_point<FORM, KT>(acc, kp, lv, j, qt, T(1));
+        for (uint32_t j = 0; j < cnt; ++j) {
+          add_point<FORM, KT>(acc, kp, lv, j, qt, T(1));
+          if constexpr (FORM == 1) {
+            if ((j & (kFlush - 1)) == kFlush - 1) acc.flush();
+          }
+        }
+        if constexpr (FORM == 1) acc.flush();
       }
     }
     fit_step<FORM, MLOD_SEM_REF_CPU>(acc, qd, qf, stepMax, nout);
@@ -721,9 +789,10 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
                                                        double4* __restrict__ qdp, BatchOut bo, uint32_t prio) {
   static_assert(PBS == kBatchWG, "one workgroup per pool block");
   using T = typename FormT<FORM>::T;
-  __shared__ uint32_t sNode[kLevels][PBS];
+  __shared__ uint32_t sFirst[kLevels][PBS];
   __shared__ uint8_t sMask[kLevels][PBS];
   __shared__ T sMul[kLevels][PBS];
+  __shared__ T sDist[kLevels][PBS];
   if (prio) __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x;
   const uint32_t lb = xcd_block();
@@ -736,7 +805,7 @@ __global__ __launch_bounds__(PBS) void k_project_batch(const ReqDesc* __restrict
     const KParams& kp = R.kp;
     const float cutoff = R.cutoff, stepMax = R.stepMax;
     const uint32_t minDepth = R.minDepth, nIter = R.nIter;
-    const LevelState<T> st{sNode, sMask, sMul};
+    const LevelState<T> st{sFirst, sMask, sMul, sDist};
     V3<double> qd;
     if constexpr (FORM == 2) {
       const double4 d4 = qdp[i];
@@ -787,11 +856,13 @@ void launch_batch_all(const ReqDesc* r, const uint2* blocks, uint32_t nblk, floa
 }
 
 template <int FORM, int SEM, int KT, bool CNT>
-void launch_one(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax,
-                unsigned long long* counters, hipStream_t s) {
+int launch_one(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax,
+               unsigned long long* counters, hipStream_t s) {
   const uint32_t m = t->qs.m;
+  if (m == 0) return MLOD_OK;
   k_project<FORM, SEM, KT, CNT><<<(m + PBS - 1) / PBS, PBS, 0, s>>>(t->view(), kp, t->qs.q, t->qs.qn, t->qs.perm, m, limit,
                                                                      nIter, stepMax, t->cutoff, t->min_depth, counters);
+  return MLOD_OK;
 }
 
 template <int FORM, int KT>
@@ -802,11 +873,11 @@ void launch_global(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32
 }
 
 template <int FORM, int SEM>
-void dispatch_kt(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax, hipStream_t s) {
+int dispatch_kt(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t nIter, float stepMax, hipStream_t s) {
   switch (kp.type) {
-    case MLOD_KERNEL_GAUSSIAN: launch_one<FORM, SEM, MLOD_KERNEL_GAUSSIAN, false>(t, kp, limit, nIter, stepMax, nullptr, s); break;
-    case MLOD_KERNEL_SINGULAR: launch_one<FORM, SEM, MLOD_KERNEL_SINGULAR, false>(t, kp, limit, nIter, stepMax, nullptr, s); break;
-    default: launch_one<FORM, SEM, MLOD_KERNEL_GAUSSIAN_MULTIPLE, false>(t, kp, limit, nIter, stepMax, nullptr, s); break;
+    case MLOD_KERNEL_GAUSSIAN: return launch_one<FORM, SEM, MLOD_KERNEL_GAUSSIAN, false>(t, kp, limit, nIter, stepMax, nullptr, s);
+    case MLOD_KERNEL_SINGULAR: return launch_one<FORM, SEM, MLOD_KERNEL_SINGULAR, false>(t, kp, limit, nIter, stepMax, nullptr, s);
+    default: return launch_one<FORM, SEM, MLOD_KERNEL_GAUSSIAN_MULTIPLE, false>(t, kp, limit, nIter, stepMax, nullptr, s);
   }
 }
 
@@ -880,20 +951,22 @@ int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hi
     return MLOD_OK;
   }
   const bool cpu = t->semantics == MLOD_SEM_REF_CPU;
+  int r;
   switch (t->precision) {
     case MLOD_PREC_FP32C:
-      if (cpu) dispatch_kt<1, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s);
-      else dispatch_kt<1, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
+      r = cpu ? dispatch_kt<1, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s)
+              : dispatch_kt<1, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
       break;
     case MLOD_PREC_FP64:
-      if (cpu) dispatch_kt<2, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s);
-      else dispatch_kt<2, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
+      r = cpu ? dispatch_kt<2, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s)
+              : dispatch_kt<2, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
       break;
     default:
-      if (cpu) dispatch_kt<0, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s);
-      else dispatch_kt<0, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
+      r = cpu ? dispatch_kt<0, MLOD_SEM_REF_CPU>(t, kp, limit, n_iter, step_max, s)
+              : dispatch_kt<0, MLOD_SEM_REF_GPU>(t, kp, limit, n_iter, step_max, s);
       break;
   }
+  if (r != MLOD_OK) return r;
   LAUNCH_CHECK();
   return MLOD_OK;
 }
@@ -903,11 +976,12 @@ int launch_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hi
 int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream_t s) {
   const KParams kp = make_kparams(t->kernel);
   const bool cpu = t->semantics == MLOD_SEM_REF_CPU;
+  int r;
 #define MLOD_CNT_CASE(SEMV)                                                                                  \
   switch (kp.type) {                                                                                         \
-    case MLOD_KERNEL_GAUSSIAN: launch_one<1, SEMV, MLOD_KERNEL_GAUSSIAN, true>(t, kp, n, 1, t->radius, d_out4, s); break; \
-    case MLOD_KERNEL_SINGULAR: launch_one<1, SEMV, MLOD_KERNEL_SINGULAR, true>(t, kp, n, 1, t->radius, d_out4, s); break; \
-    default: launch_one<1, SEMV, MLOD_KERNEL_GAUSSIAN_MULTIPLE, true>(t, kp, n, 1, t->radius, d_out4, s); break;     \
+    case MLOD_KERNEL_GAUSSIAN: r = launch_one<1, SEMV, MLOD_KERNEL_GAUSSIAN, true>(t, kp, n, 1, t->radius, d_out4, s); break; \
+    case MLOD_KERNEL_SINGULAR: r = launch_one<1, SEMV, MLOD_KERNEL_SINGULAR, true>(t, kp, n, 1, t->radius, d_out4, s); break; \
+    default: r = launch_one<1, SEMV, MLOD_KERNEL_GAUSSIAN_MULTIPLE, true>(t, kp, n, 1, t->radius, d_out4, s); break;     \
   }
   if (cpu) {
     MLOD_CNT_CASE(MLOD_SEM_REF_CPU)
@@ -915,6 +989,7 @@ int launch_count(mlod_tree* t, uint32_t n, unsigned long long* d_out4, hipStream
     MLOD_CNT_CASE(MLOD_SEM_REF_GPU)
   }
 #undef MLOD_CNT_CASE
+  if (r != MLOD_OK) return r;
   LAUNCH_CHECK();
   return MLOD_OK;
 }

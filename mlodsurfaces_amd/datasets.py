@@ -194,8 +194,12 @@ def make_workload(name: str, scale: float = 1.0):
         # capped at 16M: they need the GPU, so the caller computes them from `dc_grid` once the
         # octree is built.  `queries` holds, for callers without the GPU, the depth-11 cells of
         # [-1, 1]^3 enclosing the points (generateFromPoints, upsample.cpp:128-144).
+        # The cloud is taken in the reference's normalised frame (the application normalises every
+        # cloud it loads, Pn.cpp:262-290), so the adaptive octree's [-1, 1]^3 frame holds it and
+        # the grid reaches depth 11.
         n = max(64, int(20_000_000 * scale))
         p, nr = superquadric(n, seed=5, holes=10)
+        p = pn_normalize(p)
         cap = max(64, int(16_000_000 * scale))
         q = cell_centres_of_points(p, 11, cap=cap, seed=55)
         R = bbox_radius(p)

@@ -148,14 +148,15 @@ def test_full_size_small_configs(oracle_built, name):
     assert err.max() < 1e-6 and oerr.max() < 1e-6
 
 
-@pytest.mark.timeout(400)
+@pytest.mark.timeout(600)
 def test_full_size_c5_neighbourhoods(oracle_built):
-    """kNN (k = 32) and ball (r = 3 sigma_0) APSS at the C5 size (20M points, queries from the
-    dual-contouring grid): for a sample of queries, one FP64 step on the GPU equals the reference's
-    fit (the oracle's FP64 projectOnAPSSStats) over the neighbourhood an exact k-d tree finds, with
-    unit-area weights of the reference's multi-Gaussian kernel at the raw points
-    (CPUUtils.cpp:690-737).  The library returns float32 positions, so the two agree to that
-    rounding (~1e-8 x diag); one neighbour more or less moves a fit by ~1e-5 x diag."""
+    """kNN (k = 32) and ball (r = 3 sigma_0) APSS at the C5 size (20M points in the reference's
+    normalised frame, queries from the depth-11 dual-contouring grid): for a sample of queries, one
+    step on the GPU equals the reference's fit (the oracle's FP64 projectOnAPSSStats) over the
+    neighbourhood an exact k-d tree finds, with unit-area weights of the reference's
+    multi-Gaussian kernel at the raw points (CPUUtils.cpp:690-737): the FP64 form to the float32
+    rounding of its output (~1e-8 x diag; one neighbour more or less moves a fit by ~1e-5 x diag),
+    the compensated FP32C form (the configuration's arithmetic) within 1e-6 x diag."""
     from scipy.spatial import cKDTree
     from mlodsurfaces_amd import dc
     w = ds.make_workload("c5_knn")
@@ -165,6 +166,8 @@ def test_full_size_c5_neighbourhoods(oracle_built):
     ap = A.Fastapss(t, K)
     ap.setScalingProtectionSphere(w["cutoff"])
     Q = dc.grid_centres(ap, 11)
+    print(f"C5 DC grid at depth 11: {len(Q)} cell centres", flush=True)
+    assert len(Q) > 8_000_000
     q = Q[np.random.default_rng(11).choice(len(Q), 150, replace=False)]
     P64, N64 = P.astype(np.float64), N.astype(np.float64)
     kd = cKDTree(P64)
@@ -187,12 +190,14 @@ def test_full_size_c5_neighbourhoods(oracle_built):
             t.knnMode(True, 32)
         else:
             t.ballMode(True, -1.0)
-        g = A.Fastapss(t, K)
-        g.setScalingProtectionSphere(w["cutoff"])
-        g.set_semantics("ref_cpu")
-        g.set_precision("fp64")
-        gp, _ = g.project_points(q, 1)
-        errs = []
+        got = {}
+        for prec in ("fp64", "fp32c"):
+            g = A.Fastapss(t, K)
+            g.setScalingProtectionSphere(w["cutoff"])
+            g.set_semantics("ref_cpu")
+            g.set_precision(prec)
+            got[prec], _ = g.project_points(q, 1)
+        errs, errc = [], []
         for i, qi in enumerate(q.astype(np.float64)):
             if mode == "knn":
                 dist, j = kd.query(qi, 33)
@@ -203,8 +208,10 @@ def test_full_size_c5_neighbourhoods(oracle_built):
                 j = np.asarray(kd.query_ball_point(qi, r), np.int64)
                 if len(j) < 10:  # a fit on a few points is decided by rounding
                     continue
-            errs.append(np.linalg.norm(gp[i] - fit(qi, j)) / diag)
-        errs = np.array(errs)
-        print(f"C5 {mode}: {len(errs)} queries, GPU FP64 vs the exact neighbourhood's fit: median {np.median(errs):.1e}, "
-              f"max {errs.max():.1e} x diag")
-        assert len(errs) > 100 and errs.max() < 1e-7
+            ref = fit(qi, j)
+            errs.append(np.linalg.norm(got["fp64"][i] - ref) / diag)
+            errc.append(np.linalg.norm(got["fp32c"][i] - ref) / diag)
+        errs, errc = np.array(errs), np.array(errc)
+        print(f"C5 {mode}: {len(errs)} queries, GPU vs the exact neighbourhood's fit: FP64 median {np.median(errs):.1e} "
+              f"max {errs.max():.1e}, FP32C median {np.median(errc):.1e} max {errc.max():.1e} x diag", flush=True)
+        assert len(errs) > 100 and errs.max() < 1e-7 and errc.max() < 1e-6
