@@ -134,6 +134,17 @@ int mlod_queries_fit_kinds(mlod_tree* t, uint32_t n, uint8_t* out, void* stream)
 int mlod_project_points(mlod_tree* t, const float* xyz, uint32_t m, uint32_t n_iter, float step_max, float* out_pts,
                         float* out_nmls, void* stream);
 
+/* No counterpart in the reference.  mlod_project_points for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the points and results are device
+ * arrays (m x 3 floats each; an output may be NULL) and m <= MLOD_NOALLOC_MAX_QUERIES (larger m
+ * fails with MLOD_ERR_INVALID).  The call allocates nothing: its working buffers are a fixed
+ * 16 MiB device array per GPU, held until the process ends, so calls that overlap on one GPU
+ * race (calls on one stream are ordered).  It only launches work on `stream`: the results are
+ * in place once the stream has completed.  Same results as mlod_project_points. */
+#define MLOD_NOALLOC_MAX_QUERIES 262144u
+int mlod_project_points_noalloc(mlod_tree* t, const float* d_xyz, uint32_t m, uint32_t n_iter, float step_max,
+                                float* d_out_pts, float* d_out_nmls, void* stream);
+
 /* APSS::reorganizePoints (GPUUtils.cu:436-450): for i = 0 .. n-1 in order, the uploaded queries
  * invalid[i] and end_valid - 1 - i (caller's indices) exchange positions; normals stay with their
  * index, as in the reference (which exchanges positions only). */

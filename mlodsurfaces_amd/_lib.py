@@ -76,6 +76,7 @@ SIGNATURES = {
     "mlod_server_memory": (C.c_int, [_vp, _u64]),
     "mlod_server_cancel": (C.c_int, [_vp, C.c_uint64]),
     "mlod_project_points": (C.c_int, [_vp, _f, C.c_uint32, C.c_uint32, C.c_float, _f, _f, _vp]),
+    "mlod_project_points_noalloc": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_float, _vp, _vp, _vp]),
     "mlod_queries_reorganize": (C.c_int, [_vp, _u, C.c_uint32, C.c_uint32, _vp]),
     "mlod_queries_fit_kinds": (C.c_int, [_vp, C.c_uint32, _vp, _vp]),
     "mlod_generator_create": (C.c_int, [_vp, C.POINTER(_vp)]),

@@ -22,7 +22,8 @@ ARCH = os.environ.get("MLOD_ARCH", "gfx950")
 # Packed-FP32 instructions are kept out of all device code (-packed-fp32-ops).
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-Wno-unused-result"]
-SOURCES = ["build.hip", "project.hip", "queries.hip", "neighbors.hip", "serve.hip", "adaptive.hip", "dc.hip", "api.cpp"]
+SOURCES = ["build.hip", "project.hip", "queries.hip", "neighbors.hip", "serve.hip", "adaptive.hip", "dc.hip", "noalloc.hip",
+           "api.cpp"]
 # The octree build reproduces the reference's float arithmetic (cell boxes, radii, areas,
 # plain moments) bit for bit, and the neighbour search the kd-tree's float distances, so no
 # multiply-add may be fused there.

@@ -190,7 +190,9 @@ int mlod_queries_erase(mlod_tree* t) {
   return MLOD_OK;
 }
 
-namespace {
+}  // extern "C"
+
+namespace mlod {
 // n_iter steps of the first n uploaded queries in the tree's neighbourhood mode.  kNN / ball
 // APSS take no step bound: their fit is called with q aliased to the output
 // (CPUUtils.cpp:735), which makes the clamp a no-op.
@@ -199,7 +201,9 @@ int project_uploaded(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, 
   return (t->mode == MLOD_MODE_KNN || t->mode == MLOD_MODE_BALL) ? launch_neighbors(t, n, n_iter, s)
                                                                  : launch_project(t, n, n_iter, step_max, s);
 }
-}  // namespace
+}  // namespace mlod
+
+extern "C" {
 
 int mlod_project(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, float* out_pts, float* out_nmls, void* stream) {
   if (!t) return fail(MLOD_ERR_INVALID, "tree is NULL");

@@ -195,7 +195,26 @@ int launch_project_batch(const ReqDesc* d_reqs, const uint2* d_blocks, uint32_t 
 // neighbors.hip
 int launch_neighbors(mlod_tree* t, uint32_t n, uint32_t n_iter, hipStream_t s);
 
+// api.cpp: n_iter steps of the first n queries of t->qs in the tree's neighbourhood mode
+// (launches only)
+int project_uploaded(mlod_tree* t, uint32_t n, uint32_t n_iter, float step_max, hipStream_t s);
+
 // queries.hip
+// Buffers of the query sort: Morton codes (two arrays of m), caller indices (m) and rocPRIM's
+// temporary storage (tmpBytes, at least queries_sort_bytes(m)).
+struct SortBufs {
+  uint32_t* code;
+  uint32_t* code2;
+  uint32_t* idx;
+  void* tmp;
+  size_t tmpBytes;
+};
+size_t queries_sort_bytes(uint32_t m);  // 0 if the size query fails
+// Morton order of m device points in the tree's box: q / qn (m float4 each) receive the sorted
+// points and zero normals, perm (m) the caller index of each sorted slot.  Launches only (no
+// allocation, no synchronisation).
+int queries_order(const mlod_tree* t, const float* d_xyz, uint32_t m, float4* q, float4* qn, uint32_t* perm,
+                  const SortBufs& sb, hipStream_t s);
 int queries_set(mlod_tree* t, const float* xyz, uint32_t m, bool device_src, hipStream_t s);
 int queries_get(mlod_tree* t, uint32_t n, float* out_pts, float* out_nmls, hipStream_t s);
 int queries_scatter(mlod_tree* t, uint32_t n, float* d_pts, float* d_nmls, hipStream_t s);

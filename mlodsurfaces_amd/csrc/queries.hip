@@ -132,37 +132,50 @@ int queries_set(mlod_tree* t, const float* xyz, uint32_t m, bool device_src, hip
   qs.m = m;
   if (m == 0) return MLOD_OK;
   const float* src = xyz;
-  float* tmp = nullptr;
   if (!device_src) {
     MLOD_HIP(hipMemcpyAsync(qs.stage, xyz, (size_t)m * 3 * sizeof(float), hipMemcpyHostToDevice, s));
     src = qs.stage;
   }
   uint32_t *code = nullptr, *code2 = nullptr, *idx = nullptr;
   void* sortTmp = nullptr;
-  size_t sortBytes = 0;
-  MLOD_HIP(rocprim::radix_sort_pairs(nullptr, sortBytes, code, code2, idx, qs.perm, m, 0, 30, s));
+  const size_t sortBytes = queries_sort_bytes(m);
+  if (sortBytes == 0) return fail(MLOD_ERR_HIP, "query sort: temporary storage size query failed");
   if (hipMalloc((void**)&code, (size_t)m * 4) != hipSuccess || hipMalloc((void**)&code2, (size_t)m * 4) != hipSuccess ||
-      hipMalloc((void**)&idx, (size_t)m * 4) != hipSuccess || hipMalloc(&sortTmp, sortBytes + 16) != hipSuccess) {
+      hipMalloc((void**)&idx, (size_t)m * 4) != hipSuccess || hipMalloc(&sortTmp, sortBytes) != hipSuccess) {
     (void)hipFree(code), (void)hipFree(code2), (void)hipFree(idx), (void)hipFree(sortTmp);
     return fail(MLOD_ERR_OOM, "query sort buffers: hipMalloc failed");
   }
+  const SortBufs sb{code, code2, idx, sortTmp, sortBytes};
+  int r = queries_order(t, src, m, qs.q, qs.qn, qs.perm, sb, s);
+  hipError_t e = r == MLOD_OK ? hipStreamSynchronize(s) : hipSuccess;
+  (void)hipFree(code), (void)hipFree(code2), (void)hipFree(idx), (void)hipFree(sortTmp);
+  if (r != MLOD_OK) return r;
+  if (e != hipSuccess) return hip_fail(e, "query upload", __FILE__, __LINE__);
+  return MLOD_OK;
+}
+
+size_t queries_sort_bytes(uint32_t m) {
+  size_t bytes = 0;
+  uint32_t* none = nullptr;
+  if (rocprim::radix_sort_pairs(nullptr, bytes, none, none, none, none, m, 0, 30) != hipSuccess) return 0;
+  return bytes + 16;
+}
+
+int queries_order(const mlod_tree* t, const float* d_xyz, uint32_t m, float4* q, float4* qn, uint32_t* perm,
+                  const SortBufs& sb, hipStream_t s) {
+  if (m == 0) return MLOD_OK;
   float sc[3];
   for (int a = 0; a < 3; ++a) {
     const float ext = t->BB[a] - t->bb[a];
     sc[a] = ext > 0.f ? 1024.f / ext : 0.f;
   }
   const uint32_t nb = (m + BS - 1) / BS;
-  k_morton<<<nb, BS, 0, s>>>(src, m, t->bb[0], t->bb[1], t->bb[2], sc[0], sc[1], sc[2], code, idx);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = rocprim::radix_sort_pairs(sortTmp, sortBytes, code, code2, idx, qs.perm, m, 0, 30, s);
-  if (e == hipSuccess) {
-    k_gather<<<nb, BS, 0, s>>>(src, qs.perm, m, qs.q, qs.qn);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(code), (void)hipFree(code2), (void)hipFree(idx), (void)hipFree(sortTmp);
-  (void)tmp;
-  if (e != hipSuccess) return hip_fail(e, "query upload", __FILE__, __LINE__);
+  k_morton<<<nb, BS, 0, s>>>(d_xyz, m, t->bb[0], t->bb[1], t->bb[2], sc[0], sc[1], sc[2], sb.code, sb.idx);
+  MLOD_HIP(hipGetLastError());
+  size_t bytes = sb.tmpBytes;
+  MLOD_HIP(rocprim::radix_sort_pairs(sb.tmp, bytes, sb.code, sb.code2, sb.idx, perm, m, 0, 30, s));
+  k_gather<<<nb, BS, 0, s>>>(d_xyz, perm, m, q, qn);
+  MLOD_HIP(hipGetLastError());
   return MLOD_OK;
 }
 
