@@ -48,12 +48,16 @@ typedef struct {
   float sigmas[MLOD_MAX_SIGMAS];
 } mlod_kernel;
 
-/* APSSOctree::build options (CPUUtils.h:176-180). */
+/* APSSOctree::build options (CPUUtils.h:176-180).  Fields not set in an initializer are 0. */
 typedef struct {
   uint32_t max_depth;        /* <= 12 (MAX_DEPTH, CPUUtils.h:20) */
-  uint32_t max_pts_per_leaf; /* the GPU build supports 1, the reference default */
+  uint32_t max_pts_per_leaf; /* maxNumberOfPointsPerLeaf: a node holding at most this many points
+                                is a leaf (CPUUtils.cpp:279-283); the reference default is 1 */
   int32_t tie_policy;        /* exact-midpoint ties: 0 = low child, 1 = high child
                                 (the reference draws rand(), CPUUtils.cpp:299-308) */
+  uint32_t use_root_box;     /* updateRootBoundingBox = false (CPUUtils.cpp:547): the root's box is
+                                root_box instead of the points' bounding box */
+  float root_box[6];         /* {bb.x, bb.y, bb.z, BB.x, BB.y, BB.z}, finite, bb <= BB */
 } mlod_build_opts;
 
 /* Which of the reference's two projection functions to compute (they differ,

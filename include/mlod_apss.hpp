@@ -91,16 +91,32 @@ class APSSOctree {
   APSSOctree& operator=(const APSSOctree&) = delete;
   ~APSSOctree() { mlod_tree_destroy(m_tree); }
 
-  // APSSOctree::build (CPUUtils.h:176-180); exact-midpoint ties go to tie_policy's child.
+  // APSSOctree::build (CPUUtils.h:176-180); exact-midpoint ties go to tie_policy's child.  With
+  // updateRootBoundingBox = false the root keeps its box (CPUUtils.cpp:547): the previous build's,
+  // or the one given to setRootBoundingBox.
   template <class V>
   void build(const std::vector<V>& i_points, const std::vector<V>& normals, bool updateRootBoundingBox = true,
              unsigned maxDepth = 12, unsigned maxNumberOfPointsPerLeaf = 1, int tie_policy = 0) {
-    if (!updateRootBoundingBox) throw Error(MLOD_ERR_UNSUPPORTED, "a fixed root bounding box is not supported");
     if (i_points.size() != normals.size()) throw Error(MLOD_ERR_INVALID, "points and normals differ in length");
+    mlod_build_opts o{maxDepth, maxNumberOfPointsPerLeaf, tie_policy, 0u, {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}};
+    if (!updateRootBoundingBox) {
+      if (!m_haveBox)
+        throw Error(MLOD_ERR_STATE, "updateRootBoundingBox = false needs a root box (setRootBoundingBox or a previous build)");
+      o.use_root_box = 1u;
+      for (int a = 0; a < 6; ++a) o.root_box[a] = m_box[a];
+    }
     mlod_tree_destroy(m_tree);
     m_tree = nullptr;
-    const mlod_build_opts o{maxDepth, maxNumberOfPointsPerLeaf, tie_policy};
     check(mlod_tree_create(fptr(i_points.data()), fptr(normals.data()), (uint32_t)i_points.size(), &o, nullptr, &m_tree));
+    const mlod_tree_info i = info();
+    for (int a = 0; a < 6; ++a) m_box[a] = i.bbox[a];
+    m_haveBox = true;
+  }
+  // The root box a build with updateRootBoundingBox = false keeps (octreeNode::setBoundingBox,
+  // CPUUtils.h:118).
+  void setRootBoundingBox(const float bb[3], const float BB[3]) {
+    for (int a = 0; a < 3; ++a) m_box[a] = bb[a], m_box[3 + a] = BB[a];
+    m_haveBox = true;
   }
 
   mlod_tree_info info() const {
@@ -137,6 +153,8 @@ class APSSOctree {
  private:
   friend class APSS;
   mlod_tree* m_tree = nullptr;
+  float m_box[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  bool m_haveBox = false;
   float m_scalingProtectionSphere = 2.0f;
   unsigned m_minDepth = 0;
   bool m_knnMode = false;

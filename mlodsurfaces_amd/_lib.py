@@ -35,7 +35,8 @@ class KernelC(C.Structure):
 
 
 class BuildOptsC(C.Structure):
-    _fields_ = [("max_depth", C.c_uint32), ("max_pts_per_leaf", C.c_uint32), ("tie_policy", C.c_int32)]
+    _fields_ = [("max_depth", C.c_uint32), ("max_pts_per_leaf", C.c_uint32), ("tie_policy", C.c_int32),
+                ("use_root_box", C.c_uint32), ("root_box", C.c_float * 6)]
 
 
 class TreeInfoC(C.Structure):

@@ -108,6 +108,7 @@ class APSSOctree:
     def __init__(self):
         self._h = None
         self._n = 0
+        self._root_box = None
         self.m_scalingProtectionSphere = 2.0
         self.m_minDepth = 0
         self.m_knnMode = False
@@ -115,31 +116,53 @@ class APSSOctree:
         self.m_ballMode = False
         self.m_ballRadius = 0.1
 
+    def setRootBoundingBox(self, bb, BB):
+        """The root box a build with updateRootBoundingBox=False keeps (the reference's root node
+        box, octreeNode::setBoundingBox, CPUUtils.h:118)."""
+        self._root_box = np.concatenate([np.asarray(bb, np.float32).reshape(3), np.asarray(BB, np.float32).reshape(3)])
+
+    def _opts(self, updateRootBoundingBox, maxDepth, maxNumberOfPointsPerLeaf, tie_policy):
+        opts = _lib.BuildOptsC(int(maxDepth), int(maxNumberOfPointsPerLeaf), int(tie_policy))
+        if not updateRootBoundingBox:
+            box = getattr(self, "_root_box", None)
+            if box is None:
+                raise _lib.MlodError(_lib.ERR_STATE, "updateRootBoundingBox=False needs a root box "
+                                                     "(setRootBoundingBox or a previous build)")
+            opts.use_root_box = 1
+            for a in range(6):
+                opts.root_box[a] = float(box[a])
+        return opts
+
+    def _built(self, h, n):
+        self._h = h
+        self._n = int(n)
+        self._root_box = np.asarray(self.info()["bbox"], np.float32)
+
     def build(self, points, normals, updateRootBoundingBox=True, maxDepth=12, maxNumberOfPointsPerLeaf=1, tie_policy=0,
               stream=None):
-        if not updateRootBoundingBox:
-            raise _lib.MlodError(-4, "a fixed root bounding box is not supported")
+        """APSSOctree::build (CPUUtils.h:176-180, CPUUtils.cpp:540-569).  With
+        updateRootBoundingBox=False the root keeps its box: the one of this octree's previous
+        build, or the one given to setRootBoundingBox."""
         p = _f32(points, "points")
         n = _f32(normals, "normals")
         if len(p) != len(n):
             raise ValueError("points and normals differ in length")
+        opts = self._opts(updateRootBoundingBox, maxDepth, maxNumberOfPointsPerLeaf, tie_policy)
         self.free()
-        opts = _lib.BuildOptsC(int(maxDepth), int(maxNumberOfPointsPerLeaf), int(tie_policy))
         h = C.c_void_p()
         check(lib().mlod_tree_create(_fp(p), _fp(n), len(p), C.byref(opts), stream, C.byref(h)))
-        self._h = h
-        self._n = len(p)
+        self._built(h, len(p))
         return self
 
-    def build_device(self, d_points, d_normals, n, maxDepth=12, tie_policy=0, stream=None):
+    def build_device(self, d_points, d_normals, n, maxDepth=12, tie_policy=0, stream=None, updateRootBoundingBox=True,
+                     maxNumberOfPointsPerLeaf=1):
         """Builds from device pointers (n x 3 float32 each)."""
+        opts = self._opts(updateRootBoundingBox, maxDepth, maxNumberOfPointsPerLeaf, tie_policy)
         self.free()
-        opts = _lib.BuildOptsC(int(maxDepth), 1, int(tie_policy))
         h = C.c_void_p()
         check(lib().mlod_tree_create_device(C.c_void_p(d_points), C.c_void_p(d_normals), int(n), C.byref(opts), stream,
                                             C.byref(h)))
-        self._h = h
-        self._n = int(n)
+        self._built(h, n)
         return self
 
     @property

@@ -76,12 +76,16 @@ int mlod_tree_create_device(const float* d_xyz, const float* d_nrm, uint32_t n, 
   if (n == 0) return fail(MLOD_ERR_INVALID, "empty point set (APSSOctree::build asserts n > 0, CPUUtils.cpp:546)");
   if (!d_xyz || !d_nrm) return fail(MLOD_ERR_INVALID, "points or normals are NULL");
   if (n > (1u << 27) - 1) return fail(MLOD_ERR_UNSUPPORTED, "more than 2^27 - 1 points");
-  mlod_build_opts o{12, 1, 0};
+  mlod_build_opts o{12, 1, 0, 0, {0, 0, 0, 0, 0, 0}};
   if (opts) o = *opts;
   if (o.max_depth < 1 || o.max_depth > (uint32_t)kMaxDepth)
     return fail(MLOD_ERR_INVALID, "max_depth must be in 1..12 (MAX_DEPTH, CPUUtils.h:20)");
-  if (o.max_pts_per_leaf != 1) return fail(MLOD_ERR_UNSUPPORTED, "the GPU build supports max_pts_per_leaf = 1");
   if (o.tie_policy != 0 && o.tie_policy != 1) return fail(MLOD_ERR_INVALID, "tie_policy must be 0 or 1");
+  if (o.use_root_box) {
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(o.root_box[a]) || !std::isfinite(o.root_box[3 + a]) || o.root_box[a] > o.root_box[3 + a])
+        return fail(MLOD_ERR_INVALID, "root_box must be finite with bb <= BB on every axis");
+  }
   mlod_tree* t = new mlod_tree();
   t->n = n;
   t->opts = o;

@@ -1,7 +1,8 @@
 // build.hip — octree build, point areas and node moments on the GPU (gfx950).
 //
 // Restates APSSOctree::build (sources/apss/CPUUtils.cpp:540-569) without recursion:
-//   1. root box = float min/max of the points (computeBoundingBox, CPUUtils.cpp:571-579);
+//   1. root box = float min/max of the points (computeBoundingBox, CPUUtils.cpp:571-579), or the
+//      caller's box (updateRootBoundingBox = false, CPUUtils.cpp:547);
 //   2. every point descends 12 levels on its own: per level and axis the split is
 //      (float)(lo + hi) / 2 (the float sum of CPUUtils.cpp:286-288), the child bit is
 //      p > split (CPUUtils.cpp:298-308; exact ties follow tie_policy), the child box
@@ -9,12 +10,15 @@
 //      c = xi + 2 yi + 4 zi (CPUUtils.cpp:310).  The 12 slots form a 36-bit path code;
 //   3. a stable radix sort of (code, input index) puts the points in leaf order with
 //      input order inside a leaf (the stable partition of CPUUtils.cpp:294-311);
-//   4. with l(i, j) the number of leading levels two sorted codes share, point i is a
-//      leaf alone below depth L(i) = min(maxDepth, max(l(i-1,i), l(i,i+1)) + 1) (a node
-//      is a leaf iff it holds <= 1 point or sits at maxDepth, CPUUtils.cpp:279-283), and
-//      it is the first point of the nodes at depths l(i-1,i)+1 .. L(i).  Ranking those
-//      "owned" depths per depth gives the breadth-first index (depth, Morton prefix),
-//      which is the order of index_nodes_by_breadth_first_search (CPUUtils.cpp:601-619);
+//   4. with l(i, j) the number of leading levels two sorted codes share, a node is a leaf iff
+//      it holds at most M = maxNumberOfPointsPerLeaf points or sits at maxDepth
+//      (CPUUtils.cpp:279-283).  The depth-d node holding point i holds more than M points iff
+//      some window of M + 1 consecutive sorted points containing i shares d levels, so i's
+//      leaf is at depth L(i) = min(maxDepth, W(i) + 1) with W(i) the largest l(j, j + M) over
+//      j = i - M .. i (for M = 1, max(l(i-1,i), l(i,i+1))).  Point i is the first point of the
+//      nodes at depths l(i-1,i)+1 .. L(i).  Ranking those "owned" depths per depth gives the
+//      breadth-first index (depth, Morton prefix), which is the order of
+//      index_nodes_by_breadth_first_search (CPUUtils.cpp:601-619);
 //   5. areas = (leaf box diagonal)^2 / #leaf points (estimateAreasRecursive, CPUUtils.cpp:236-252);
 //   6. node moments bottom-up, one depth per launch: plain FP32 sums in the reference's
 //      order (leaf points in input order, children in slot order, CPUUtils.cpp:358-398),
@@ -154,24 +158,88 @@ struct PointSpan {
   int endLo; // first depth whose segment ends at i (l(i,i+1) + 1)
 };
 
-__device__ __forceinline__ PointSpan point_span(const uint64_t* __restrict__ codes, uint32_t i, uint32_t n, int maxDepth) {
+// W: the window maxima W(i) for maxNumberOfPointsPerLeaf != 1, or NULL (M = 1).
+__device__ __forceinline__ PointSpan point_span(const uint64_t* __restrict__ codes, uint32_t i, uint32_t n, int maxDepth,
+                                                const int8_t* __restrict__ W) {
   const uint64_t c = codes[i];
   const int lL = i > 0 ? lcp_levels(codes[i - 1], c, maxDepth) : -1;
   const int lR = i + 1 < n ? lcp_levels(c, codes[i + 1], maxDepth) : -1;
   PointSpan s;
-  s.L = min(maxDepth, max(lL, lR) + 1);
+  s.L = min(maxDepth, (W ? (int)W[i] : max(lL, lR)) + 1);
   s.lo = lL + 1;
   s.endLo = lR + 1;
   return s;
 }
 
-__global__ __launch_bounds__(BS) void k_count(const uint64_t* __restrict__ codes, uint32_t n, int maxDepth, uint32_t nb,
-                                              uint32_t* __restrict__ blockCnt) {
+// A[j] = l(j, j + M), -1 where j + M >= n.
+__global__ __launch_bounds__(BS) void k_window_lcp(const uint64_t* __restrict__ codes, uint32_t n, uint32_t M,
+                                                   int maxDepth, int8_t* __restrict__ A) {
+  const uint32_t j = blockIdx.x * BS + threadIdx.x;
+  if (j >= n) return;
+  A[j] = (uint64_t)j + M < n ? (int8_t)lcp_levels(codes[j], codes[j + M], maxDepth) : (int8_t)-1;
+}
+
+// Running maxima of A forward (P) and backward (S) inside blocks of B = M + 1 entries, one
+// thread per block: a window of B entries spans at most two blocks, so its maximum is
+// max(S[first], P[last]).
+__global__ __launch_bounds__(BS) void k_block_max(const int8_t* __restrict__ A, uint32_t n, uint64_t B,
+                                                  int8_t* __restrict__ P, int8_t* __restrict__ S) {
+  const uint64_t lo = ((uint64_t)blockIdx.x * BS + threadIdx.x) * B;
+  if (lo >= n) return;
+  const uint64_t hi = lo + B < n ? lo + B : n;
+  int m = -1;
+  for (uint64_t j = lo; j < hi; ++j) {
+    m = max(m, (int)A[j]);
+    P[j] = (int8_t)m;
+  }
+  m = -1;
+  for (uint64_t j = hi; j-- > lo;) {
+    m = max(m, (int)A[j]);
+    S[j] = (int8_t)m;
+  }
+}
+
+// Leaf order for maxNumberOfPointsPerLeaf != 1: a leaf above the last level holds points whose
+// codes differ below it, and the reference keeps a leaf's points in input order (the stable
+// partition of CPUUtils.cpp:294-311).  key = (code cut at the point's leaf depth) << 27 | input
+// index, unique, so a sort by it lists the leaves in the same order with their points by index.
+__global__ __launch_bounds__(BS) void k_leaf_keys(const uint64_t* __restrict__ codes, const uint32_t* __restrict__ order,
+                                                  const int8_t* __restrict__ W, uint32_t n, int maxDepth,
+                                                  uint64_t* __restrict__ key, uint32_t* __restrict__ pos) {
+  const uint32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const int L = min(maxDepth, (int)W[i] + 1);
+  const uint64_t keep = L <= 0 ? 0ull : (((1ull << (3 * kMaxDepth)) - 1ull) & ~((1ull << (3 * (kMaxDepth - L))) - 1ull));
+  key[i] = ((codes[i] & keep) << 27) | order[i];
+  pos[i] = i;
+}
+
+// the leaf-ordered codes, input indices and window maxima
+__global__ __launch_bounds__(BS) void k_leaf_apply(const uint64_t* __restrict__ key, const uint32_t* __restrict__ pos,
+                                                   const int8_t* __restrict__ W, uint32_t n, uint64_t* __restrict__ codes,
+                                                   uint32_t* __restrict__ order, int8_t* __restrict__ W2) {
+  const uint32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  codes[i] = key[i] >> 27;
+  order[i] = (uint32_t)(key[i] & ((1u << 27) - 1u));
+  W2[i] = W[pos[i]];
+}
+
+// W(i) = max of A over j = max(0, i - M) .. i
+__global__ __launch_bounds__(BS) void k_window_max(const int8_t* __restrict__ P, const int8_t* __restrict__ S, uint32_t n,
+                                                   uint32_t M, int8_t* __restrict__ W) {
+  const uint32_t i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  W[i] = (uint64_t)i >= M ? (int8_t)max((int)S[i - M], (int)P[i]) : P[i];
+}
+
+__global__ __launch_bounds__(BS) void k_count(const uint64_t* __restrict__ codes, uint32_t n, int maxDepth,
+                                              const int8_t* __restrict__ W, uint32_t nb, uint32_t* __restrict__ blockCnt) {
   __shared__ uint32_t wt[kMaxDepth + 1][BS / 64];
   const uint32_t i = blockIdx.x * BS + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   PointSpan sp{1, 0, 1};
-  if (i < n) sp = point_span(codes, i, n, maxDepth);
+  if (i < n) sp = point_span(codes, i, n, maxDepth, W);
 #pragma unroll
   for (int d = 0; d <= kMaxDepth; ++d) {
     const bool own = i < n && d <= maxDepth && sp.lo <= d && d <= sp.L;
@@ -220,7 +288,8 @@ struct LevelOff {
 // Writes every node owned by a point (the point is the first of the node's range) and
 // every node whose range ends at the point.
 __global__ __launch_bounds__(BS) void k_emit(const uint64_t* __restrict__ codes, const uint32_t* __restrict__ order,
-                                             const float* __restrict__ xyz, uint32_t n, int maxDepth, uint32_t nb,
+                                             const float* __restrict__ xyz, uint32_t n, int maxDepth,
+                                             const int8_t* __restrict__ W, uint32_t nb,
                                              const uint32_t* __restrict__ blockOff, LevelOff lev,
                                              const float* __restrict__ box6, float4* __restrict__ geo,
                                              uint4* __restrict__ hdr, float* __restrict__ box,
@@ -232,7 +301,7 @@ __global__ __launch_bounds__(BS) void k_emit(const uint64_t* __restrict__ codes,
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool valid = i < n;
   PointSpan sp{1, 0, 1};
-  if (valid) sp = point_span(codes, i, n, maxDepth);
+  if (valid) sp = point_span(codes, i, n, maxDepth, W);
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t rank[kMaxDepth + 1];
   bool own[kMaxDepth + 1];
@@ -544,13 +613,19 @@ int build_tree(mlod_tree* t, const float* d_xyz, const float* d_nrm, hipStream_t
   const uint32_t nbb = std::min<uint32_t>(1024u, (n + BS - 1) / BS);
   float *part = nullptr, *dbox = nullptr;
   if ((r = sc.alloc(&part, 6 * (size_t)nbb)) || (r = sc.alloc(&dbox, 6))) return r;
-  k_bbox_partial<<<nbb, BS, 0, s>>>(d_xyz, n, part);
-  LAUNCH_CHECK();
-  k_bbox_final<<<1, BS, 0, s>>>(part, nbb, dbox);
-  LAUNCH_CHECK();
   float hbox[6];
-  MLOD_HIP(hipMemcpyAsync(hbox, dbox, sizeof(hbox), hipMemcpyDeviceToHost, s));
-  MLOD_HIP(hipStreamSynchronize(s));
+  if (t->opts.use_root_box) {
+    std::memcpy(hbox, t->opts.root_box, sizeof(hbox));
+    MLOD_HIP(hipMemcpyAsync(dbox, hbox, sizeof(hbox), hipMemcpyHostToDevice, s));
+    MLOD_HIP(hipStreamSynchronize(s));
+  } else {
+    k_bbox_partial<<<nbb, BS, 0, s>>>(d_xyz, n, part);
+    LAUNCH_CHECK();
+    k_bbox_final<<<1, BS, 0, s>>>(part, nbb, dbox);
+    LAUNCH_CHECK();
+    MLOD_HIP(hipMemcpyAsync(hbox, dbox, sizeof(hbox), hipMemcpyDeviceToHost, s));
+    MLOD_HIP(hipStreamSynchronize(s));
+  }
   for (int a = 0; a < 3; ++a) {
     t->bb[a] = hbox[a];
     t->BB[a] = hbox[3 + a];
@@ -573,7 +648,39 @@ int build_tree(mlod_tree* t, const float* d_xyz, const float* d_nrm, hipStream_t
   if ((r = sc.alloc(&blockCnt, (size_t)(kMaxDepth + 1) * nb)) || (r = sc.alloc(&blockOff, (size_t)(kMaxDepth + 1) * nb)) ||
       (r = sc.alloc(&levelTot, kMaxDepth + 1)))
     return r;
-  k_count<<<nb, BS, 0, s>>>(codes2, n, maxDepth, nb, blockCnt);
+  // leaf depths for maxNumberOfPointsPerLeaf != 1: window maxima of l(j, j + M)
+  int8_t* W = nullptr;
+  const uint32_t M = t->opts.max_pts_per_leaf;
+  if (M != 1) {
+    int8_t *A = nullptr, *P = nullptr, *S = nullptr;
+    if ((r = sc.alloc(&A, n)) || (r = sc.alloc(&P, n)) || (r = sc.alloc(&S, n)) || (r = sc.alloc(&W, n))) return r;
+    k_window_lcp<<<nb, BS, 0, s>>>(codes2, n, M, maxDepth, A);
+    LAUNCH_CHECK();
+    const uint64_t B = (uint64_t)M + 1;
+    const uint64_t nblk = ((uint64_t)n + B - 1) / B;
+    k_block_max<<<(uint32_t)((nblk + BS - 1) / BS), BS, 0, s>>>(A, n, B, P, S);
+    LAUNCH_CHECK();
+    k_window_max<<<nb, BS, 0, s>>>(P, S, n, M, W);
+    LAUNCH_CHECK();
+    // re-sort into leaf order (points of a leaf by input index)
+    uint64_t *key = nullptr, *key2 = nullptr;
+    uint32_t *pos = nullptr, *pos2 = nullptr;
+    int8_t* W2 = nullptr;
+    if ((r = sc.alloc(&key, n)) || (r = sc.alloc(&key2, n)) || (r = sc.alloc(&pos, n)) || (r = sc.alloc(&pos2, n)) ||
+        (r = sc.alloc(&W2, n)))
+      return r;
+    k_leaf_keys<<<nb, BS, 0, s>>>(codes2, t->order, W, n, maxDepth, key, pos);
+    LAUNCH_CHECK();
+    size_t tb = 0;
+    MLOD_HIP(rocprim::radix_sort_pairs(nullptr, tb, key, key2, pos, pos2, n, 0, 3 * kMaxDepth + 27, s));
+    void* tmp2 = nullptr;
+    if ((r = sc.alloc((char**)&tmp2, tb))) return r;
+    MLOD_HIP(rocprim::radix_sort_pairs(tmp2, tb, key, key2, pos, pos2, n, 0, 3 * kMaxDepth + 27, s));
+    k_leaf_apply<<<nb, BS, 0, s>>>(key2, pos2, W, n, codes2, t->order, W2);
+    LAUNCH_CHECK();
+    W = W2;
+  }
+  k_count<<<nb, BS, 0, s>>>(codes2, n, maxDepth, W, nb, blockCnt);
   LAUNCH_CHECK();
   k_scan_levels<<<kMaxDepth + 1, 1024, 0, s>>>(blockCnt, blockOff, nb, levelTot);
   LAUNCH_CHECK();
@@ -609,7 +716,7 @@ int build_tree(mlod_tree* t, const float* d_xyz, const float* d_nrm, hipStream_t
       (r = dalloc(&t->span, nn, &t->bytes)))
     return r;
   MLOD_HIP(hipMemsetAsync(dties, 0, sizeof(unsigned long long), s));
-  k_emit<<<nb, BS, 0, s>>>(codes2, t->order, d_xyz, n, maxDepth, nb, blockOff, lev, dbox, t->geo, t->hdr, t->box, nodeLast,
+  k_emit<<<nb, BS, 0, s>>>(codes2, t->order, d_xyz, n, maxDepth, W, nb, blockOff, lev, dbox, t->geo, t->hdr, t->box, nodeLast,
                            pointLeaf, reinterpret_cast<uint32_t*>(t->span), dties);
   LAUNCH_CHECK();
   k_finalize<<<(nn + BS - 1) / BS, BS, 0, s>>>(nn, t->hdr, nodeLast);

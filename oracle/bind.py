@@ -76,7 +76,7 @@ def oracle_lib():
             raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
         L = C.CDLL(ORACLE_SO)
         L.orc_tree_build.restype = _vp
-        L.orc_tree_build.argtypes = [_f, _f, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32]
+        L.orc_tree_build.argtypes = [_f, _f, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32, _f]
         L.orc_tree_free.argtypes = [_vp]
         L.orc_tree_num_nodes.restype = C.c_uint32
         L.orc_tree_num_nodes.argtypes = [_vp]
@@ -130,7 +130,7 @@ def ref_lib():
             raise RuntimeError(f"{REF_SO} missing: run `make -C oracle ref` where the reference sources exist")
         L = C.CDLL(REF_SO)
         L.ref_tree_build.restype = _vp
-        L.ref_tree_build.argtypes = [_f, _f, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.ref_tree_build.argtypes = [_f, _f, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f]
         L.ref_tree_free.argtypes = [_vp]
         L.ref_tree_num_nodes.restype = C.c_uint32
         L.ref_tree_num_nodes.argtypes = [_vp]
@@ -156,12 +156,14 @@ def _c32(a):
 class OracleTree:
     """The oracle's restated octree (CPUUtils.cpp:540-619)."""
 
-    def __init__(self, points, normals, max_depth=12, max_pts_per_leaf=1, tie_policy=0, seed=12345):
+    def __init__(self, points, normals, max_depth=12, max_pts_per_leaf=1, tie_policy=0, seed=12345, root_box=None):
         self.L = oracle_lib()
         self.p = _c32(points)
         self.n = _c32(normals)
+        self.box = None if root_box is None else np.ascontiguousarray(root_box, np.float32).reshape(6)
         self.h = self.L.orc_tree_build(_ptr(self.p, C.c_float), _ptr(self.n, C.c_float), len(self.p), max_depth,
-                                       max_pts_per_leaf, tie_policy, seed)
+                                       max_pts_per_leaf, tie_policy, seed,
+                                       None if self.box is None else _ptr(self.box, C.c_float))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -251,12 +253,13 @@ def oracle_fit(sums9, q, step_max=1e30, ref_gpu=False):
 class RefTree:
     """The reference's own CPU APSSOctree (compiled into oracle/_ref)."""
 
-    def __init__(self, points, normals, max_depth=12, max_pts_per_leaf=1, seed=12345):
+    def __init__(self, points, normals, max_depth=12, max_pts_per_leaf=1, seed=12345, root_box=None):
         self.L = ref_lib()
         self.p = _c32(points)
         self.n = _c32(normals)
+        self.box = None if root_box is None else np.ascontiguousarray(root_box, np.float32).reshape(6)
         self.h = self.L.ref_tree_build(_ptr(self.p, C.c_float), _ptr(self.n, C.c_float), len(self.p), max_depth,
-                                       max_pts_per_leaf, seed)
+                                       max_pts_per_leaf, seed, None if self.box is None else _ptr(self.box, C.c_float))
 
     def __del__(self):
         if getattr(self, "h", None):

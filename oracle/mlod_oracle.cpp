@@ -304,7 +304,10 @@ static void momentsRec(Tree& t, int id, const std::vector<Mom<T>>& pts, std::vec
 
 // APSSOctree::build, CPUUtils.cpp:540-569 (computeBoundingBox 571-579, estimateAreas 581-587,
 // buildapssNodeStats 589-599, index_nodes_by_breadth_first_search 601-619, getRelations 213-234)
-static Tree* build(const float* xyz, const float* nrm, uint32_t n, uint32_t maxDepth, uint32_t maxPts, int tie_policy) {
+// box6: a fixed root box {bb, BB} (build with updateRootBoundingBox = false, CPUUtils.cpp:547), or
+// NULL for the points' bounding box.
+static Tree* build(const float* xyz, const float* nrm, uint32_t n, uint32_t maxDepth, uint32_t maxPts, int tie_policy,
+                   const float* box6) {
   Tree* t = new Tree();
   t->maxDepth = maxDepth;
   t->maxPts = maxPts;
@@ -315,13 +318,17 @@ static Tree* build(const float* xyz, const float* nrm, uint32_t n, uint32_t maxD
     t->P[i] = mk<float>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
     t->N[i] = mk<float>(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
   }
-  for (int a = 0; a < 3; ++a) t->bb[a] = t->BB[a] = (&t->P[0].x)[a];
-  for (uint32_t i = 1; i < n; ++i)
-    for (int a = 0; a < 3; ++a) {
-      const float v = (&t->P[i].x)[a];
-      t->bb[a] = std::min(t->bb[a], v);
-      t->BB[a] = std::max(t->BB[a], v);
-    }
+  if (box6) {
+    for (int a = 0; a < 3; ++a) t->bb[a] = box6[a], t->BB[a] = box6[3 + a];
+  } else {
+    for (int a = 0; a < 3; ++a) t->bb[a] = t->BB[a] = (&t->P[0].x)[a];
+    for (uint32_t i = 1; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        const float v = (&t->P[i].x)[a];
+        t->bb[a] = std::min(t->bb[a], v);
+        t->BB[a] = std::max(t->BB[a], v);
+      }
+  }
   t->nodes.reserve(2 * (size_t)n + 16);
   t->nodes.emplace_back();
   for (int a = 0; a < 3; ++a) {
@@ -607,10 +614,10 @@ static void projectOne(const Tree& t, const orc_kernel& k, const orc_params& prm
 extern "C" {
 
 void* orc_tree_build(const float* xyz, const float* nrm, uint32_t n, uint32_t max_depth, uint32_t max_pts_per_leaf,
-                     int32_t tie_policy, uint32_t seed) {
+                     int32_t tie_policy, uint32_t seed, const float* box6) {
   if (n == 0) return nullptr;
   if (tie_policy == 2) srand(seed);  // the reference's rand() stream (ref_harness seeds it the same way)
-  return orc::build(xyz, nrm, n, max_depth, max_pts_per_leaf, tie_policy);
+  return orc::build(xyz, nrm, n, max_depth, max_pts_per_leaf, tie_policy, box6);
 }
 void orc_tree_free(void* h) { delete static_cast<orc::Tree*>(h); }
 uint32_t orc_tree_num_nodes(void* h) { return (uint32_t) static_cast<orc::Tree*>(h)->nodes.size(); }

@@ -42,7 +42,10 @@ typedef struct {
   float ball_radius;
 } ref_params;
 
-void* ref_tree_build(const float* xyz, const float* nrm, uint32_t n, uint32_t max_depth, uint32_t max_pts, uint32_t seed) {
+// box6: a fixed root box {bb, BB}: it is set on the root node and the tree is built with
+// updateRootBoundingBox = false (CPUUtils.cpp:540-547); NULL builds with the points' box.
+void* ref_tree_build(const float* xyz, const float* nrm, uint32_t n, uint32_t max_depth, uint32_t max_pts, uint32_t seed,
+                     const float* box6) {
   if (n == 0) return nullptr;
   srand(seed);  // exact-midpoint ties draw rand() (CPUUtils.cpp:299-308)
   std::vector<glm::vec3> P(n), N(n);
@@ -51,7 +54,13 @@ void* ref_tree_build(const float* xyz, const float* nrm, uint32_t n, uint32_t ma
     N[i] = glm::vec3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
   }
   RefTree* t = new RefTree();
-  t->build(P, N, true, max_depth, max_pts);
+  if (box6) {
+    // the reference offers no setter for the root box; its root node is created non-const
+    BBOX b;
+    b.set(glm::vec3(box6[0], box6[1], box6[2]), glm::vec3(box6[3], box6[4], box6[5]));
+    const_cast<octreeNode*>(t->get_root())->setBoundingBox(b);
+  }
+  t->build(P, N, box6 == nullptr, max_depth, max_pts);
   return t;
 }
 

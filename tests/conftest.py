@@ -36,6 +36,31 @@ def golden_kernel(g):
 MODE_NAMES = {0: "mlod", 1: "global", 2: "knn", 3: "ball"}
 
 
+def oracle_tree_opts(t):
+    """APSSOctree::build options recorded in a tree_* fixture, as OracleTree / RefTree keywords."""
+    o = dict(max_depth=int(t.get("max_depth", 12)), max_pts_per_leaf=int(t.get("max_pts", 1)))
+    if "root_box" in t:
+        o["root_box"] = t["root_box"]
+    return o
+
+
+def gpu_tree_opts(t):
+    """The same options as keywords of the library's APSSOctree.build (a fixed root box is set
+    on the octree first, as setRootBoundingBox)."""
+    o = dict(maxDepth=int(t.get("max_depth", 12)), maxNumberOfPointsPerLeaf=int(t.get("max_pts", 1)))
+    if "root_box" in t:
+        o["updateRootBoundingBox"] = False
+    return o
+
+
+def gpu_tree_from_fixture(t):
+    from mlodsurfaces_amd.apss import APSSOctree
+    tr = APSSOctree()
+    if "root_box" in t:
+        tr.setRootBoundingBox(t["root_box"][:3], t["root_box"][3:])
+    return tr.build(t["points"], t["normals"], **gpu_tree_opts(t))
+
+
 @pytest.fixture(scope="session")
 def oracle_built():
     from oracle import bind

@@ -13,7 +13,8 @@ stores the inputs and the reference's outputs (no reference code) as compressed 
 
 Inputs are chosen free of exact-midpoint ties (the reference resolves those with rand(),
 CPUUtils.cpp:299-308); the tie count is checked with the oracle.  Re-run with
-    make -C oracle all ref && python tests/golden/make_golden.py
+    make -C oracle all ref && python tests/golden/make_golden.py [tree names ...]
+(names: only those trees and their projections).
 """
 from __future__ import annotations
 
@@ -46,24 +47,48 @@ def clouds():
     return out
 
 
-def tree_fixture(name, p, n):
-    r = bind.RefTree(p, n, seed=SEED)
+# Trees built with other APSSOctree::build options (CPUUtils.h:176-180): several points per leaf
+# at a lower maximum depth, and a fixed root box (updateRootBoundingBox = false).  The fixture
+# records the options (max_depth, max_pts, root_box) next to the tree.
+def option_trees():
+    cl = clouds()
+    p, n = cl["sphere5k"]
+    out = {"sphere5k_m4d8": (p, n, dict(max_depth=8, max_pts_per_leaf=4))}
+    p, n = cl["torus6k"]
+    lo, hi = p.min(0).astype(np.float64), p.max(0).astype(np.float64)
+    ext = hi - lo
+    box = np.concatenate([lo - [0.17, 0.05, 0.31] * ext, hi + [0.29, 0.11, 0.02] * ext]).astype(np.float32)
+    out["torus6k_box"] = (p, n, dict(max_pts_per_leaf=2, root_box=box))
+    return out
+
+
+def tree_fixture(name, p, n, opts=None):
+    opts = opts or {}
+    r = bind.RefTree(p, n, seed=SEED, **opts)
     d = r.nodes()
     d["areas"] = r.areas()
     d["bbox"] = r.bbox()
     d["points"] = p
     d["normals"] = n
+    d["max_depth"] = np.int32(opts.get("max_depth", 12))
+    d["max_pts"] = np.int32(opts.get("max_pts_per_leaf", 1))
+    if "root_box" in opts:
+        d["root_box"] = np.asarray(opts["root_box"], np.float32)
     np.savez_compressed(os.path.join(HERE, f"tree_{name}.npz"), **d)
     return r
 
 
-def main():
+def main(only=None):
     cases = []
-    for name, (p, n) in clouds().items():
-        o = bind.OracleTree(p, n)
+    trees = {name: (p, n, {}) for name, (p, n) in clouds().items()}
+    trees.update(option_trees())
+    for name, (p, n, opts) in trees.items():
+        if only and name not in only:
+            continue
+        o = bind.OracleTree(p, n, **opts)
         if o.ties:
             raise SystemExit(f"{name}: {o.ties} exact-midpoint ties; pick another seed")
-        r = tree_fixture(name, p, n)
+        r = tree_fixture(name, p, n, opts)
         R = ds.bbox_radius(p)
         rng = np.random.default_rng(7)
         qj = (p[rng.choice(len(p), 600, replace=False)] + rng.uniform(-0.01, 0.01, (600, 3))).astype(np.float32)
@@ -79,6 +104,8 @@ def main():
             cases.append((name, r, p, qj[:200], "global_mg4", mg4, dict(mode="global"), 1))
             cases.append((name, r, p, qj[:300], "knn20_mg4", mg4, dict(mode="knn", knn=20), 2))
             cases.append((name, r, p, qj[:300], "ball3s_mg4", mg4, dict(mode="ball", ball_radius=-1.0), 2))
+        if opts:
+            cases.append((name, r, p, qj, "mg4_c3_it3", mg4, dict(cutoff=3.0), 3))
         if name == "sq6k":
             mg6 = dict(type=2, sigmas=[0.01 * 2.0 ** i * R for i in range(6)])
             cases.append((name, r, p, qj, "mg6_c2", mg6, dict(cutoff=2.0), 1))
@@ -92,6 +119,9 @@ def main():
                             out_normals=on, n_iter=iters, cutoff=pr.cutoff, min_depth=pr.min_depth, mode=pr.mode,
                             knn=pr.knn, ball_radius=pr.ball_radius, **{f"kernel_{a}": v for a, v in kd.items()})
         print(f"proj_{name}_{tag}: {len(q)} queries")
+    if only:
+        print("done")
+        return
     # kd-tree neighbour sets
     p, n = clouds()["sphere5k"]
     r = bind.RefTree(p, n, seed=SEED)
@@ -101,4 +131,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]) or None)

@@ -8,7 +8,7 @@ output positions are float32, so one unit in the last place is the floor)."""
 import numpy as np
 import pytest
 
-from conftest import MODE_NAMES, golden_kernel, golden_names, load_golden
+from conftest import MODE_NAMES, golden_kernel, golden_names, gpu_tree_from_fixture, load_golden, oracle_tree_opts
 from mlodsurfaces_amd import datasets as ds
 
 pytestmark = pytest.mark.gpu
@@ -26,6 +26,14 @@ def tree_for(name, p, n):
     return _trees[name]
 
 
+def golden_tree(name, t):
+    """The octree of golden tree fixture `name`, built with the fixture's options."""
+    key = "golden:" + name
+    if key not in _trees:
+        _trees[key] = gpu_tree_from_fixture(t)
+    return _trees[key]
+
+
 MLOD_GOLDEN = [n for n in golden_names("proj_") if "knn" not in n and "ball" not in n]
 
 
@@ -37,7 +45,7 @@ def test_matches_reference_cpu_golden(oracle_built, name, precision):
     tn = str(g["tree"])
     t = load_golden(f"tree_{tn}.npz")
     print("case", name, precision, flush=True)
-    tree = tree_for(tn, t["points"], t["normals"])
+    tree = golden_tree(tn, t)
     mode = MODE_NAMES[int(g["mode"])]
     kern = golden_kernel(g)
     op, on, _ = gpu_project(tree, g["queries"], kern, cutoff=float(g["cutoff"]), min_depth=int(g["min_depth"]),
@@ -47,7 +55,7 @@ def test_matches_reference_cpu_golden(oracle_built, name, precision):
     assert e.max() < TOL_REF, e.max()
     assert np.all(np.abs(np.linalg.norm(on, axis=1) - 1) < 1e-4)
     # and the FP64 oracle with the same semantics
-    o = oracle_built.OracleTree(t["points"], t["normals"])
+    o = oracle_built.OracleTree(t["points"], t["normals"], **oracle_tree_opts(t))
     prm = oracle_built.make_params(cutoff=float(g["cutoff"]), min_depth=int(g["min_depth"]), semantics="ref_cpu",
                                    mode=mode)
     o64, _ = o.project(g["queries"], oracle_built.make_kernel(**kern), prm, n_iter=int(g["n_iter"]), precision=1)
@@ -84,6 +92,31 @@ def test_matches_fp64_oracle(oracle_built, semantics, kname, precision):
     e = rel_err(op, o64.astype(np.float32), 2 * R)
     assert e.max() < TOL_ORACLE[precision], (e.max(), np.sqrt(np.mean(e ** 2)))
     assert np.all(np.sum(on * n64, 1) > 0.9999)
+
+
+@pytest.mark.parametrize("max_pts,max_depth", [(4, 12), (8, 8), (32, 10)])
+@pytest.mark.parametrize("semantics", ["ref_gpu", "ref_cpu"])
+@pytest.mark.parametrize("precision", ["fp32", "fp32c", "fp64"])
+def test_multi_point_leaves_match_fp64_oracle(oracle_built, max_pts, max_depth, semantics, precision):
+    """Octrees with several points per leaf (maxNumberOfPointsPerLeaf, CPUUtils.cpp:279-283): the
+    GPU function sums every point of a leaf (Fastapss.cu:189-200), the CPU one keeps the last
+    (CPUUtils.cpp:448-457); both against the FP64 oracle with the same semantics and tree."""
+    from gpu_util import gpu_project, gpu_tree, rel_err
+    print("case", max_pts, max_depth, semantics, precision, flush=True)
+    p, n = ds.torus(40000, seed=31)
+    key = f"torus40k_m{max_pts}_d{max_depth}"
+    if key not in _trees:
+        _trees[key] = gpu_tree(p, n, maxDepth=max_depth, maxNumberOfPointsPerLeaf=max_pts)
+    tree = _trees[key]
+    R = ds.bbox_radius(p)
+    kern = KERNELS["mg4"](R)
+    q = ds.jitter_queries(p[:2000], 0.01, seed=7)
+    op, on, _ = gpu_project(tree, q, kern, cutoff=2.5, semantics=semantics, precision=precision, n_iter=2)
+    o = oracle_built.OracleTree(p, n, max_depth=max_depth, max_pts_per_leaf=max_pts)
+    o64, n64 = o.project(q, oracle_built.make_kernel(**kern), oracle_built.make_params(cutoff=2.5, semantics=semantics),
+                         n_iter=2, precision=1)
+    e = rel_err(op, o64.astype(np.float32), 2 * R)
+    assert e.max() < TOL_ORACLE[precision], (e.max(), np.sqrt(np.mean(e ** 2)))
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp32c", "fp64"])
@@ -214,7 +247,7 @@ def test_knn_ball_match_reference_golden(oracle_built, name, precision):
     tn = str(g["tree"])
     t = load_golden(f"tree_{tn}.npz")
     print("case", name, precision, flush=True)
-    tree = tree_for(tn, t["points"], t["normals"])
+    tree = golden_tree(tn, t)
     mode = MODE_NAMES[int(g["mode"])]
     kern = golden_kernel(g)
     op, on, _ = gpu_project(tree, g["queries"], kern, semantics="ref_cpu", precision=precision,
@@ -230,7 +263,7 @@ def test_knn_ball_match_reference_golden(oracle_built, name, precision):
     assert np.isfinite(op[ok]).all()
     e = rel_err(op[ok], ref[ok], diag)
     assert np.median(e) < 1e-7 and e.max() < TOL_REF, e.max()
-    o = oracle_built.OracleTree(t["points"], t["normals"])
+    o = oracle_built.OracleTree(t["points"], t["normals"], **oracle_tree_opts(t))
     prm = oracle_built.make_params(semantics="ref_cpu", mode=mode, knn=int(g["knn"]), ball_radius=float(g["ball_radius"]))
     o64, _ = o.project(g["queries"], oracle_built.make_kernel(**kern), prm, n_iter=int(g["n_iter"]), precision=1)
     ok &= np.isfinite(o64).all(1)

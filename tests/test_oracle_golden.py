@@ -3,7 +3,7 @@ oracle/_ref (the reference's CPUUtils.cpp + Kernel.cpp) must be reproduced."""
 import numpy as np
 import pytest
 
-from conftest import MODE_NAMES, golden_kernel, golden_names, load_golden
+from conftest import MODE_NAMES, golden_kernel, golden_names, load_golden, oracle_tree_opts
 
 TREE_KEYS = ["depth", "bb", "center", "radius", "father", "first_child", "next_brother", "num_children", "leaf_count",
              "leaf_offset"]
@@ -12,7 +12,7 @@ TREE_KEYS = ["depth", "bb", "center", "radius", "father", "first_child", "next_b
 @pytest.mark.parametrize("name", golden_names("tree_"))
 def test_oracle_tree_bit_exact(oracle_built, name):
     g = load_golden(name)
-    o = oracle_built.OracleTree(g["points"], g["normals"])
+    o = oracle_built.OracleTree(g["points"], g["normals"], **oracle_tree_opts(g))
     assert o.ties == 0
     d = o.nodes()
     for k in TREE_KEYS:
@@ -31,7 +31,7 @@ def test_oracle_tree_bit_exact(oracle_built, name):
 def test_oracle_projection_matches_reference(oracle_built, name):
     g = load_golden(name)
     t = load_golden(f"tree_{g['tree']}.npz")
-    o = oracle_built.OracleTree(t["points"], t["normals"])
+    o = oracle_built.OracleTree(t["points"], t["normals"], **oracle_tree_opts(t))
     k = oracle_built.make_kernel(**golden_kernel(g))
     mode = MODE_NAMES[int(g["mode"])]
     prm = oracle_built.make_params(cutoff=float(g["cutoff"]), min_depth=int(g["min_depth"]), semantics="ref_cpu",
