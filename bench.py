@@ -79,10 +79,11 @@ def cpu_baseline(w, kern, cutoff, budget_s=15.0):
     from oracle import bind
     if not bind.have_ref():
         return None
+    release = os.path.exists(bind.REF_SO_RELEASE)
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     t0 = time.time()
     log(f"cpu baseline: building the reference octree over {len(w['points'])} points")
-    tree = bind.RefTree(w["points"], w["normals"])
+    tree = bind.RefTree(w["points"], w["normals"], release=release)
     build_s = time.time() - t0
     log(f"cpu baseline: reference octree built in {build_s:.1f} s")
     k = bind.make_kernel(**kern)
@@ -100,8 +101,9 @@ def cpu_baseline(w, kern, cutoff, budget_s=15.0):
     dt = time.time() - t0
     return {"value": ns / dt, "unit": "projected pts/s", "cores": cores, "kind": "reference",
             "sample": f"first {ns} of {len(q)} queries, 1 step each, reference APSSOctree::projectCPU (REF_CPU, "
-                      f"{w.get('mode', 'mlod')}) with OpenMP on {cores} host threads; reference octree build "
-                      f"{build_s:.1f} s untimed"}
+                      f"{w.get('mode', 'mlod')}) with OpenMP on {cores} host threads, built "
+                      f"{'with the reference release flags (-O3, x86-64-v4)' if release else '-O2'}; reference octree "
+                      f"build {build_s:.1f} s untimed"}
 
 
 def adaptive_measure(A, ds, w, depth, semantics, precision):
@@ -281,17 +283,28 @@ def calibrate_copy():
     log(f"copy calibration: 3 dispatches, {n * 4} bytes read and written each")
 
 
+def kernel_source_sha():
+    """SHA-256 (first 16 hex digits) of the hot kernel's source, project.hip."""
+    import hashlib
+    with open(os.path.join(ROOT, "mlodsurfaces_amd", "csrc", "project.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
 def traffic_from_profile(kernel_name, workload):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by the factor a plain copy
-    kernel of known size reads for the same counter), or None if no summary matches."""
+    kernel of known size reads for the same counter) and whether that summary was measured on
+    the current kernel source (its `kernel_source_sha16:` line), or (None, False)."""
     path = os.path.join(ROOT, "profiles", f"{kernel_name}_{workload}.md")
     if not os.path.exists(path):
-        return None
+        return None, False
+    traffic, sha = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1].split()[0])
-    return None
+            traffic = float(line.split(":", 1)[1].split()[0])
+        if line.startswith("kernel_source_sha16:"):
+            sha = line.split(":", 1)[1].strip()
+    return traffic, sha == kernel_source_sha()
 
 
 def serve_measure(n_clouds=64, seed_rate=2000.0):
@@ -330,6 +343,43 @@ def serve_measure(n_clouds=64, seed_rate=2000.0):
     return out
 
 
+EXTRA_CONFIGS = {
+    # the other configurations' measured lines: steps, warm-up steps
+    "torus1m": (10, 2),   # 1M torus, rational kernel, with the adaptive octree at depth 10
+    "c5_knn": (3, 1),     # 20M points, kNN APSS k = 32, DC-grid queries at depth 11 (16M)
+    "c5_ball": (2, 1),    # the same with ball APSS r = 3 sigma_0
+}
+
+
+def run_extra_configs(names, timeout_s=420):
+    """Each named configuration measured by this script in a child process (started before this
+    process uses the GPU), its progress lines passed through; returns {name: summary}."""
+    import subprocess
+    out = {}
+    env = dict(os.environ, MLOD_DATA_CACHE=os.environ.get("MLOD_DATA_CACHE", "/tmp/mlod_data_cache"))
+    for name in names:
+        steps, warm = EXTRA_CONFIGS[name]
+        cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", name, "--steps", str(steps), "--warmup",
+               str(warm), "--no-cpu-baseline", "--no-serve", "--no-real-models", "--extra-configs", "none"]
+        log(f"other configuration {name}: {' '.join(cmd[2:])}")
+        t0 = time.time()
+        try:
+            p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=timeout_s, env=env)
+        except subprocess.TimeoutExpired:
+            out[name] = {"status": f"timed out after {timeout_s} s"}
+            continue
+        lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+        if p.returncode != 0 or not lines:
+            out[name] = {"status": f"exit {p.returncode}"}
+            continue
+        r = json.loads(lines[-1])
+        keep = ("value", "unit", "ms_per_step", "steps", "config", "forms", "error_vs_fp64", "counters_per_step",
+                "roofline", "adaptive_octree")
+        out[name] = {k: r.get(k) for k in keep}
+        out[name]["wall_s"] = time.time() - t0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,6 +405,9 @@ def main():
                     help="depth of the surface mesh of the workload's cloud (0: none)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain elementwise copy of 512 MiB three times (to scale the HBM counters)")
+    ap.add_argument("--extra-configs", default=",".join(EXTRA_CONFIGS),
+                    help="with the headline workload, the other configurations measured in child processes "
+                         "(comma-separated names, or 'none')")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -364,6 +417,14 @@ def main():
               "replicating the octree and splitting the Morton-ordered queries across GPUs; that is not "
               "built here.  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    other = None
+    extra = [x for x in args.extra_configs.split(",") if x and x != "none"]
+    if extra and args.workload == "superquadric8m" and args.scale == 1.0:
+        unknown = [x for x in extra if x not in EXTRA_CONFIGS]
+        if unknown:
+            raise SystemExit(f"unknown --extra-configs {unknown}")
+        other = run_extra_configs(extra)
 
     from mlodsurfaces_amd import apss as A
     from mlodsurfaces_amd import datasets as ds
@@ -476,9 +537,30 @@ def main():
                 # plane / sphere switch flips of the last step against the FP64 form
                 errs[form]["plane_sphere_flips"] = int(np.sum((r["kinds"] == 1) != (ref64["kinds"] == 1)))
                 errs[form]["plane_fits"] = int(np.sum(r["kinds"] == 1))
+            if mode == "ball":
+                # a ball holding 1-3 points gives a degenerate fit whose result is rounding noise in
+                # every arithmetic (as in the reference): the outliers are classified by their ball
+                # population (start positions; exact counts from a k-d tree over the points)
+                out_i = np.where(~ok | (np.nan_to_num(e, nan=1.0) > 1e-6))[0]
+                if len(out_i):
+                    from scipy.spatial import cKDTree
+                    if "_kd" not in w:
+                        w["_kd"] = cKDTree(P.astype(np.float64))
+                    rb = 3.0 * w["kernel"]["sigmas"][0] if w["ball_radius"] < 0 else w["ball_radius"]
+                    cnt = np.array([len(x) for x in w["_kd"].query_ball_point(Q[out_i].astype(np.float64), rb)])
+                    deg = cnt < 4
+                    keep = np.ones(len(e), bool)
+                    keep[out_i[deg]] = False
+                    e2 = e[keep & ok]
+                    errs[form].update(degenerate_balls=int(deg.sum()), max_nondegenerate=float(e2.max()),
+                                      above_1e6_nondegenerate=int(np.sum(e2 > 1e-6)),
+                                      nonfinite_nondegenerate=int(np.sum(~ok & keep)))
             log(f"{form} vs fp64 after {args.steps} steps: max {errs[form]['max']:.3e} rms {errs[form]['rms']:.3e} "
                 f"p99.99 {errs[form]['p9999']:.3e} diag, {errs[form]['above_1e6']} queries above 1e-6 diag, "
-                f"{errs[form].get('plane_sphere_flips', '-')} plane/sphere flips")
+                f"{errs[form].get('plane_sphere_flips', '-')} plane/sphere flips"
+                + (f"; outliers in balls of < 4 points: {errs[form]['degenerate_balls']}, otherwise max "
+                   f"{errs[form]['max_nondegenerate']:.3e}, {errs[form]['above_1e6_nondegenerate']} above 1e-6, "
+                   f"{errs[form]['nonfinite_nondegenerate']} non-finite" if "degenerate_balls" in errs[form] else ""))
 
     head = results[args.precision]
     mean_launch_s = float(np.mean(head["launch_ms"])) / 1e3
@@ -499,7 +581,7 @@ def main():
                 "mean_launch_ms": mean_launch_s * 1e3}
     if args.workload == "superquadric8m" and args.scale == 1.0 and args.precision == "fp32c" and \
             args.semantics == "ref_gpu":
-        roofline["traffic"] = traffic_from_profile("k_project", args.workload)
+        roofline["traffic"], roofline["traffic_profile_current"] = traffic_from_profile("k_project", args.workload)
     adaptive = None
     if args.workload == "torus1m":
         adaptive = adaptive_measure(A, ds, w, args.adaptive_depth, args.semantics, args.precision)
@@ -555,6 +637,7 @@ def main():
         "adaptive_octree": adaptive,
         "real_models": real,
         "surface_mesh": mesh,
+        "other_configs": other,
     }
     print(json.dumps(out), flush=True)
 

@@ -15,6 +15,8 @@ cast to float32.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 __all__ = ["sphere", "torus", "superquadric", "jitter_queries", "offset_queries", "make_workload"]
@@ -188,6 +190,13 @@ def make_workload(name: str, scale: float = 1.0):
         return dict(points=p, normals=nr, queries=q, kernel=dict(type=2, sigmas=sig), cutoff=2.0,
                     description=f"superquadric N=M={n}, multi-Gaussian S=6, cutoff 2, offsets +-0.02 diag")
     if name in ("c5_knn", "c5_ball"):
+        cache = os.environ.get("MLOD_DATA_CACHE")
+        path = os.path.join(cache, f"c5_{scale:g}.npz") if cache else None
+        if path and os.path.exists(path):
+            with np.load(path, allow_pickle=False) as z:
+                p, nr, q = z["p"], z["n"], z["q"]
+        else:
+            p = nr = q = None
         # global-APSS variants at scale: 20M points on the displaced superquadric with 10 holes.
         # The queries are the centres of the dual-contouring grid's finest cells (the adaptive
         # octree generated to depth 11 with the MLoD projection, uniformized; dc.grid_centres),
@@ -198,10 +207,14 @@ def make_workload(name: str, scale: float = 1.0):
         # cloud it loads, Pn.cpp:262-290), so the adaptive octree's [-1, 1]^3 frame holds it and
         # the grid reaches depth 11.
         n = max(64, int(20_000_000 * scale))
-        p, nr = superquadric(n, seed=5, holes=10)
-        p = pn_normalize(p)
         cap = max(64, int(16_000_000 * scale))
-        q = cell_centres_of_points(p, 11, cap=cap, seed=55)
+        if p is None:
+            p, nr = superquadric(n, seed=5, holes=10)
+            p = pn_normalize(p)
+            q = cell_centres_of_points(p, 11, cap=cap, seed=55)
+            if path:
+                os.makedirs(cache, exist_ok=True)
+                np.savez(path, p=p, n=nr, q=q)
         R = bbox_radius(p)
         sig = [0.01 * 2.0 ** i * R for i in range(4)]
         mode = "knn" if name == "c5_knn" else "ball"

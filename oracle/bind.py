@@ -13,6 +13,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libmlodref.so")
+# the same reference sources with the reference's release flags (the CPU baseline's build)
+REF_SO_RELEASE = os.path.join(HERE, "_ref", "libmlodref_release.so")
 
 MODES = {"mlod": 0, "global": 1, "knn": 2, "ball": 3}
 SEMANTICS = {"ref_gpu": 0, "ref_cpu": 1}
@@ -63,7 +65,7 @@ _u = C.POINTER(C.c_uint32)
 _u64 = C.POINTER(C.c_uint64)
 
 _orc = None
-_ref = None
+_ref = {}
 
 # void (*)(const float* xyz, uint32_t n, float* out_pts, float* out_nmls, void* user)
 PROJ_CB = C.CFUNCTYPE(None, _f, C.c_uint32, _f, _f, _vp)
@@ -123,12 +125,12 @@ def oracle_lib():
     return _orc
 
 
-def ref_lib():
-    global _ref
-    if _ref is None:
-        if not os.path.exists(REF_SO):
-            raise RuntimeError(f"{REF_SO} missing: run `make -C oracle ref` where the reference sources exist")
-        L = C.CDLL(REF_SO)
+def ref_lib(release=False):
+    so = REF_SO_RELEASE if release else REF_SO
+    if so not in _ref:
+        if not os.path.exists(so):
+            raise RuntimeError(f"{so} missing: run `make -C oracle ref` where the reference sources exist")
+        L = C.CDLL(so)
         L.ref_tree_build.restype = _vp
         L.ref_tree_build.argtypes = [_f, _f, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f]
         L.ref_tree_free.argtypes = [_vp]
@@ -141,8 +143,8 @@ def ref_lib():
         L.ref_project.argtypes = [_vp, C.POINTER(KernelC), C.POINTER(ParamsC), _f, C.c_uint32, C.c_uint32, C.c_float, _f,
                                   _f, C.c_int32]
         L.ref_knn.argtypes = [_vp, _f, C.c_uint32, C.c_uint32, _f]
-        _ref = L
-    return _ref
+        _ref[so] = L
+    return _ref[so]
 
 
 def have_ref():
@@ -253,8 +255,8 @@ def oracle_fit(sums9, q, step_max=1e30, ref_gpu=False):
 class RefTree:
     """The reference's own CPU APSSOctree (compiled into oracle/_ref)."""
 
-    def __init__(self, points, normals, max_depth=12, max_pts_per_leaf=1, seed=12345, root_box=None):
-        self.L = ref_lib()
+    def __init__(self, points, normals, max_depth=12, max_pts_per_leaf=1, seed=12345, root_box=None, release=False):
+        self.L = ref_lib(release)
         self.p = _c32(points)
         self.n = _c32(normals)
         self.box = None if root_box is None else np.ascontiguousarray(root_box, np.float32).reshape(6)
