@@ -72,6 +72,16 @@ enum {
   MLOD_PREC_FP64 = 2       /* plain FP64 everywhere */
 };
 
+/* How the MLoD walk is laid on the GPU (no counterpart in the reference, whose kernel walks one
+ * thread per query, Fastapss.cu:160-333).  LANE: each lane walks its own query's tree path
+ * (the default, and the faster on every workload measured).  WAVE: one walk per wavefront over
+ * the union of its 64 lanes' paths, node records in scalar registers, each lane taking part in
+ * the nodes its own walk visits (near/far decided per lane and combined by ballot).  Both add
+ * every lane's terms in the same order: the FP64 form gives the same bits, the FP32 forms the
+ * same results to their rounding (the two loops may fuse a multiply-add differently).  Serving
+ * launches use the lane walk. */
+enum { MLOD_TRAVERSAL_LANE = 0, MLOD_TRAVERSAL_WAVE = 1 };
+
 /* Neighbourhood: MLoD octree (Fastapss), all points (GlobalAPSS), k nearest or ball
  * (APSSOctree::knnMode / ballMode, CPUUtils.h:191-192). */
 enum { MLOD_MODE_MLOD = 0, MLOD_MODE_GLOBAL = 1, MLOD_MODE_KNN = 2, MLOD_MODE_BALL = 3 };
@@ -108,6 +118,9 @@ int mlod_set_kernel(mlod_tree* t, const mlod_kernel* k);
 int mlod_set_params(mlod_tree* t, float cutoff, uint32_t min_depth, int32_t semantics, int32_t precision);
 /* APSSOctree::knnMode / ballMode (CPUUtils.cpp:674-688); ball_radius < 0 => 3 * sigmas[0]. */
 int mlod_set_mode(mlod_tree* t, int32_t mode, uint32_t knn, float ball_radius);
+/* No counterpart in the reference: MLOD_TRAVERSAL_LANE (default) or MLOD_TRAVERSAL_WAVE for the
+ * MLoD projection steps of this tree (mlod_project, mlod_project_points*, mlod_count_visits). */
+int mlod_set_traversal(mlod_tree* t, int32_t traversal);
 
 /* APSS::copyPointsToGPU (GPUUtils.cu:406-411): uploads m query points (host, m x 3). */
 int mlod_queries_upload(mlod_tree* t, const float* xyz, uint32_t m, void* stream);

@@ -263,6 +263,7 @@ class APSS {
 
   void set_semantics(int semantics) { m_semantics = semantics; }
   void set_precision(int precision) { m_precision = precision; }
+  void set_traversal(int traversal) { m_traversal = traversal; }
   void set_stream(void* hip_stream) { m_stream = hip_stream; }
   const APSSOctree* CPU() const { return m_apssOctree; }
 
@@ -280,6 +281,7 @@ class APSS {
     if (mode == MLOD_MODE_MLOD && m_apssOctree->m_knnMode) mode = MLOD_MODE_KNN;
     if (mode == MLOD_MODE_MLOD && m_apssOctree->m_ballMode) mode = MLOD_MODE_BALL;
     check(mlod_set_mode(t, mode, m_apssOctree->m_knn, m_apssOctree->m_ballRadius));
+    check(mlod_set_traversal(t, m_traversal));
   }
 
   APSSOctree* m_apssOctree;
@@ -288,6 +290,7 @@ class APSS {
   unsigned m_minDepth = 0;
   int m_semantics = MLOD_SEM_REF_GPU;
   int m_precision = MLOD_PREC_FP32;
+  int m_traversal = MLOD_TRAVERSAL_LANE;
   void* m_stream = nullptr;
 };
 

@@ -59,6 +59,7 @@ SIGNATURES = {
     "mlod_set_kernel": (C.c_int, [_vp, C.POINTER(KernelC)]),
     "mlod_set_params": (C.c_int, [_vp, C.c_float, C.c_uint32, C.c_int32, C.c_int32]),
     "mlod_set_mode": (C.c_int, [_vp, C.c_int32, C.c_uint32, C.c_float]),
+    "mlod_set_traversal": (C.c_int, [_vp, C.c_int32]),
     "mlod_queries_upload": (C.c_int, [_vp, _f, C.c_uint32, _vp]),
     "mlod_queries_set_device": (C.c_int, [_vp, _vp, C.c_uint32, _vp]),
     "mlod_queries_erase": (C.c_int, [_vp]),

@@ -259,6 +259,7 @@ class APSS:
         self.m_minDepth = apssOctree.m_minDepth
         self.semantics = _lib.SEM_REF_GPU
         self.precision = _lib.PREC_FP32
+        self.traversal = 0
         self._kernel = None
         self._m = 0
         if kernel is not None:
@@ -299,6 +300,11 @@ class APSS:
         """'fp32' (plain), 'fp32c' (compensated, FP64 fit) or 'fp64'."""
         self.precision = {"fp32": 0, "fp32c": 1, "fp64": 2}.get(precision, precision)
 
+    def set_traversal(self, traversal):
+        """'lane' (one walk per query, the default) or 'wave' (one walk per wavefront over the
+        union of its queries' walks); no counterpart in the reference."""
+        self.traversal = {"lane": 0, "wave": 1}.get(traversal, traversal)
+
     def _mode(self):
         t = self.m_apssOctree
         if self.MODE == _lib.MODE_MLOD and t.m_knnMode:
@@ -313,6 +319,7 @@ class APSS:
                                     int(self.precision)))
         t = self.m_apssOctree
         check(lib().mlod_set_mode(h, self._mode(), t.m_knn, t.m_ballRadius))
+        check(lib().mlod_set_traversal(h, int(self.traversal)))
 
     # -- queries --------------------------------------------------------------------------
     def copyPointsToGPU(self, nbOfVectors, outputPoints, stream=None):

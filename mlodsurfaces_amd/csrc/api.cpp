@@ -165,6 +165,13 @@ int mlod_set_params(mlod_tree* t, float cutoff, uint32_t min_depth, int32_t sema
   return MLOD_OK;
 }
 
+int mlod_set_traversal(mlod_tree* t, int32_t traversal) {
+  if (!t) return fail(MLOD_ERR_INVALID, "tree is NULL");
+  if (traversal != MLOD_TRAVERSAL_LANE && traversal != MLOD_TRAVERSAL_WAVE) return fail(MLOD_ERR_INVALID, "bad traversal");
+  t->traversal = traversal;
+  return MLOD_OK;
+}
+
 int mlod_set_mode(mlod_tree* t, int32_t mode, uint32_t knn, float ball_radius) {
   if (!t) return fail(MLOD_ERR_INVALID, "tree is NULL");
   if (mode < MLOD_MODE_MLOD || mode > MLOD_MODE_BALL) return fail(MLOD_ERR_INVALID, "bad mode");

@@ -133,6 +133,7 @@ struct mlod_tree {
   float cutoff = 2.0f;      // m_scalingProtectionSphere default, GPUUtils.h:63
   uint32_t min_depth = 0;   // m_minDepth default, GPUUtils.h:64
   int32_t semantics = MLOD_SEM_REF_GPU;
+  int32_t traversal = MLOD_TRAVERSAL_LANE;
   int32_t precision = MLOD_PREC_FP32;
   int32_t mode = MLOD_MODE_MLOD;
   uint32_t knn = 20;

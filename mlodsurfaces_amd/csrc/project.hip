@@ -561,6 +561,229 @@ __device__ __forceinline__ void traverse(typename FormT<FORM>::Acc& acc, const K
   if constexpr (FORM == 1) acc.flush();
 }
 
+// ----------------------------------------------------------------------------- wave-shared walk
+// The same MLoD sum, walked once per wavefront over the union of its lanes' walks.  The node
+// being handled is wave-uniform, so its records come through scalar loads; each lane keeps
+// its own multiplier, distance term, share sum and near-child bits and takes part only in the
+// nodes its own walk visits (a lane is active at a child when it was active at the parent and
+// found the child near).  The union walk is a depth-first walk in child order, like each lane's
+// own walk, so every lane adds the same terms in the same order as walking alone, and it counts
+// its own iterations (one per tested child and one per finished node), so the compensated
+// form's block flushes fall where they fall in a lane's own walk.
+struct WaveLevel {
+  uint32_t first;  // the waiting node's first child
+  uint32_t rest;   // its children still to expand (near for at least one lane)
+  uint64_t act;    // the lanes active at the waiting node
+};
+
+template <class T>
+struct WaveStack {
+  WaveLevel (*lv)[PBS / 64];  // per level and wavefront
+  T (*mul)[PBS];              // per level and lane: the waiting node's multiplier
+  T (*dist)[PBS];             // its distance term (negative: no blending below it)
+  uint8_t (*near)[PBS];       // its near children for this lane
+};
+
+// A wave-uniform record through the constant address space: the tree is not written while a
+// projection runs, so the load is a scalar load into SGPRs (one per wavefront, through the
+// scalar cache) instead of a vector load per lane.
+template <class T>
+__device__ __forceinline__ T ld_uni(const T* base, uint32_t idx) {
+  static_assert(sizeof(T) % 4 == 0, "records are whole dwords");
+  typedef uint32_t W __attribute__((ext_vector_type(sizeof(T) / 4)));
+  const W w = *(const __attribute__((address_space(4))) W*)((const __attribute__((address_space(4))) char*)base +
+                                                            (uint64_t)idx * sizeof(T));
+  T r;
+  __builtin_memcpy(&r, &w, sizeof(T));
+  return r;
+}
+
+template <int FORM>
+__device__ __forceinline__ NodeMom<FORM> load_mom_uni(const TreeView& tv, uint32_t c) {
+  NodeMom<FORM> r;
+  if constexpr (FORM == 0) {
+    r.m0 = ld_uni(tv.momP, 2 * c), r.m1 = ld_uni(tv.momP, 2 * c + 1);
+  } else if constexpr (FORM == 1) {
+    r.m0 = ld_uni(tv.momC, 2 * c), r.m1 = ld_uni(tv.momC, 2 * c + 1);
+  } else {
+    r.m0 = ld_uni(tv.momD, 2 * c), r.m1 = ld_uni(tv.momD, 2 * c + 1);
+    r.A = ld_uni(tv.AD, c);
+  }
+  return r;
+}
+
+template <int FORM, int SEM, int KT, bool CNT>
+__device__ __forceinline__ void traverse_wave(typename FormT<FORM>::Acc& acc, const KParams& kp, const TreeView& tv,
+                                              V3<typename FormT<FORM>::T> q, typename FormT<FORM>::T cutoff,
+                                              uint32_t minDepth, bool live, WaveStack<typename FormT<FORM>::T> st,
+                                              int tid, int wv, Counters& ct, uint32_t& waveIters) {
+  using T = typename FormT<FORM>::T;
+  const uint32_t lane = __lane_id();
+  uint32_t itc = 0;  // the lane's own iterations since the last block flush (compensated form)
+  const uint4 h0 = ld_uni(tv.hdr, 0u);
+  if (hdr_leaf(h0.y)) {
+    if (live) add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, h0, q, T(1), ct);
+    return;
+  }
+  const float4 g0 = ld_uni(tv.geo, 0u);
+  bool blendNode = minDepth == 0;
+  bool act = live;
+  T m = T(1), distP = T(-1), shareSum = T(0);
+  uint32_t nearb = 0;
+  if (act) {
+    const T d0 = tdist<FORM>(q, g0);
+    if (minDepth == 0 && d0 > (T)g0.w * cutoff) {
+      add_mom<FORM, KT>(acc, kp, load_mom_uni<FORM>(tv, 0u), __uint_as_float(h0.z), q, T(1));
+      if constexpr (CNT) ct.far_ += 1;
+      act = false;
+    } else {
+      distP = blendNode ? fabs(d0 - (T)g0.w * cutoff) : T(-1);
+    }
+  }
+  uint64_t actMask = __ballot(act);
+  if (!actMask) return;
+  // the current node X (wave-uniform)
+  uint32_t first = h0.x, nc = hdr_count(h0.y), k = 0, un = 0, pend = 0;
+  int lvl = 0;
+  float Ai = __uint_as_float(h0.z);
+  T shareScale = SEM == MLOD_SEM_REF_CPU ? tratio<FORM>(T(1), (T)nc) : tratio<FORM>(T(1), (T)Ai);
+  NodeMom<FORM> momX = load_mom_uni<FORM>(tv, 0);
+  while (true) {
+    if constexpr (CNT) ++waveIters;
+    // one iteration: test child k of X, or finish X (its own (1 - b_c) share); either way at
+    // most one node term per lane, added at the one call site below
+    const bool testing = k < nc;
+    NodeMom<FORM> mm = momX;
+    float At = Ai;
+    T f = m * shareSum;
+    bool contrib = false;
+    if (testing) {
+      const uint32_t x = __builtin_amdgcn_readfirstlane(first + k);
+      const float4 gx = ld_uni(tv.geo, x);
+      const uint4 hx = ld_uni(tv.hdr, x);
+      mm = load_mom_uni<FORM>(tv, x);
+      const float Ax = __uint_as_float(hx.z);
+      const bool leaf = hdr_leaf(hx.y);
+      const bool single = leaf && hdr_count(hx.y) == 1u;
+      const bool deep = hdr_depth(hx.y) >= minDepth;
+      bool nearc = false;
+      if (act) {
+        if constexpr (CNT) ct.iters += 1, ct.tested += 1;
+        const T dxq = tdist<FORM>(q, gx);
+        const T rc = (T)gx.w * cutoff;
+        const T dp = blendNode ? distP : T(-1);
+        T beta = T(1);
+        if (dp >= T(0)) {
+          const T dc = fmax(dxq - rc, T(0));
+          beta = blend(tratio<FORM>(dp, dp + dc));
+        }
+        const T mx = m * beta;
+        if (blendNode) {
+          const T share = SEM == MLOD_SEM_REF_CPU ? shareScale : (FORM == 1 ? (T)Ax * shareScale : (T)Ax / (T)Ai);
+          shareSum += (T(1) - beta) * share;
+        }
+        const bool far = !leaf && deep && dxq > rc;
+        if (leaf && !single) add_leaf<FORM, SEM, KT, CNT>(acc, kp, tv, hx, q, mx, ct);
+        nearc = !leaf && !far;
+        if (nearc) nearb |= 1u << k;
+        if constexpr (CNT) {
+          ct.far_ += far ? 1u : 0u;
+          ct.leafPts += single ? 1u : 0u;
+        }
+        contrib = far || single;
+        f = mx;
+      }
+      At = Ax;
+      if (__ballot(nearc)) un |= 1u << k;
+      ++k;
+    } else if (act) {
+      if constexpr (CNT) {
+        ct.iters += 1;
+        ct.nearN += (blendNode && shareSum != T(0)) ? 1u : 0u;
+      }
+      contrib = blendNode && shareSum != T(0);
+    }
+    if (contrib) add_mom<FORM, KT>(acc, kp, mm, At, q, f);
+    if constexpr (FORM == 1) {
+      if (act && ++itc == kFlush) {
+        acc.flush();
+        itc = 0;
+      }
+    }
+    if (testing) continue;
+    // X is finished: the next node to expand
+    uint32_t firstP, c, nbits;
+    uint64_t actP;
+    T mp, dp;
+    if (un) {
+      // X's first near child is next; the rest wait at this level
+      c = __builtin_ctz(un);
+      const uint32_t rest = un & (un - 1u);
+      mp = m;
+      dp = blendNode ? distP : T(-1);
+      nbits = nearb;
+      actP = actMask;
+      firstP = first;
+      if (rest) {
+        if (lane == 0) st.lv[lvl][wv] = WaveLevel{first, rest, actMask};
+        st.mul[lvl][tid] = mp;
+        st.dist[lvl][tid] = dp;
+        st.near[lvl][tid] = (uint8_t)nearb;
+        pend |= 1u << lvl;
+      }
+      lvl = lvl + 1;
+    } else if (pend) {
+      // the deepest waiting node's next near child
+      const int L = 31 - __builtin_clz(pend);
+      const WaveLevel e = st.lv[L][wv];
+      firstP = __builtin_amdgcn_readfirstlane(e.first);
+      const uint32_t wm = __builtin_amdgcn_readfirstlane(e.rest);
+      actP = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(e.act >> 32)) << 32) |
+             (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)e.act);
+      c = __builtin_ctz(wm);
+      const uint32_t rest = wm & (wm - 1u);
+      if (rest) {
+        if (lane == 0) st.lv[L][wv].rest = rest;
+      } else {
+        pend &= ~(1u << L);
+      }
+      mp = st.mul[L][tid];
+      dp = st.dist[L][tid];
+      nbits = st.near[L][tid];
+      lvl = L + 1;
+    } else {
+      break;
+    }
+    // descend: the child becomes X for the lanes that found it near
+    act = ((actP >> lane) & 1u) && ((nbits >> c) & 1u);
+    actMask = __ballot(act);
+    const uint32_t xn = __builtin_amdgcn_readfirstlane(firstP + c);
+    const float4 gx = ld_uni(tv.geo, xn);
+    const uint4 hx = ld_uni(tv.hdr, xn);
+    momX = load_mom_uni<FORM>(tv, xn);
+    if (act) {
+      const T dxq = tdist<FORM>(q, gx);
+      const T rc = (T)gx.w * cutoff;
+      T beta = T(1);
+      if (dp >= T(0)) {
+        const T dc = fmax(dxq - rc, T(0));
+        beta = blend(tratio<FORM>(dp, dp + dc));
+      }
+      m = mp * beta;
+      distP = fabs(dxq - rc);
+    }
+    first = hx.x;
+    nc = hdr_count(hx.y);
+    k = 0;
+    un = 0;
+    nearb = 0;
+    shareSum = T(0);
+    blendNode = hdr_depth(hx.y) >= minDepth;
+    Ai = __uint_as_float(hx.z);
+    shareScale = SEM == MLOD_SEM_REF_CPU ? tratio<FORM>(T(1), (T)nc) : tratio<FORM>(T(1), (T)Ai);
+  }
+}
+
 // ----------------------------------------------------------------------------- sphere fit
 // Algebraic sphere fit and projection, Fastapss.cu:354-395 (REF_GPU: plane below |u4| 1e-6,
 // normalize(0) = 0) and CPUUtils.cpp:739-782 (REF_CPU: plane below 1e-12, normalize(0) = NaN).
@@ -721,6 +944,58 @@ __global__ __launch_bounds__(PBS) void k_project(TreeView tv, KParams kp, float4
   }
 }
 
+// k_project with the wave-shared walk: every lane of a wavefront calls the walk (lanes without a
+// query take part in no node), so the walk's control flow stays wave-uniform.
+template <int FORM, int SEM, int KT, bool CNT>
+__global__ __launch_bounds__(PBS) void k_project_wave(TreeView tv, KParams kp, float4* __restrict__ q,
+                                                      float4* __restrict__ qn, const uint32_t* __restrict__ perm,
+                                                      uint32_t m, uint32_t limit, uint32_t nIter, float stepMax,
+                                                      float cutoff, uint32_t minDepth,
+                                                      unsigned long long* __restrict__ counters) {
+  using T = typename FormT<FORM>::T;
+  __shared__ WaveLevel sLv[kLevels][PBS / 64];
+  __shared__ T sMul[kLevels][PBS];
+  __shared__ T sDist[kLevels][PBS];
+  __shared__ uint8_t sNear[kLevels][PBS];
+  const int tid = threadIdx.x;
+  const int wv = tid >> 6;
+  const uint32_t i = xcd_block() * PBS + tid;
+  const bool live = i < m && !(limit < m && perm[i] >= limit);
+  Counters ct{0, 0, 0, 0, 0};
+  uint32_t waveIters = 0;
+  const WaveStack<T> st{sLv, sMul, sDist, sNear};
+  const float4 q4 = live ? q[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  V3<double> qd{q4.x, q4.y, q4.z};
+  float4 nout = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t it = 0; it < nIter; ++it) {
+    typename FormT<FORM>::Acc acc;
+    acc.init();
+    const V3<float> qf{(float)qd.x, (float)qd.y, (float)qd.z};
+    V3<T> qt;
+    if constexpr (FORM == 2)
+      qt = V3<T>{qd.x, qd.y, qd.z};
+    else
+      qt = V3<T>{qf.x, qf.y, qf.z};
+    traverse_wave<FORM, SEM, KT, CNT>(acc, kp, tv, qt, (T)cutoff, minDepth, live, st, tid, wv, ct, waveIters);
+    if constexpr (FORM == 1) acc.flush();
+    if (live) fit_step<FORM, SEM>(acc, qd, qf, stepMax, nout);
+  }
+  if (live) {
+    q[i] = make_float4((float)qd.x, (float)qd.y, (float)qd.z, 0.f);
+    qn[i] = nout;
+  }
+  if constexpr (CNT) {
+    if (live) {
+      atomicAdd(&counters[0], (unsigned long long)ct.far_);
+      atomicAdd(&counters[1], (unsigned long long)ct.leafPts);
+      atomicAdd(&counters[2], (unsigned long long)ct.tested);
+      atomicAdd(&counters[3], (unsigned long long)ct.nearN);
+      atomicAdd(&counters[4], (unsigned long long)ct.iters);
+    }
+    if (__lane_id() == 0) atomicAdd(&counters[5], (unsigned long long)waveIters);
+  }
+}
+
 // Global APSS over all points (GlobalAPSS.cu:113-192 / APSSStats, CPUUtils.cpp:515-528):
 // point tiles staged through LDS, every lane of the workgroup sums the tile for its query.
 // The fit uses the 1e-12 plane threshold of GlobalAPSS.cu:143 (the REF_CPU fit).
@@ -860,8 +1135,13 @@ int launch_one(const mlod_tree* t, const KParams& kp, uint32_t limit, uint32_t n
                unsigned long long* counters, hipStream_t s) {
   const uint32_t m = t->qs.m;
   if (m == 0) return MLOD_OK;
-  k_project<FORM, SEM, KT, CNT><<<(m + PBS - 1) / PBS, PBS, 0, s>>>(t->view(), kp, t->qs.q, t->qs.qn, t->qs.perm, m, limit,
-                                                                     nIter, stepMax, t->cutoff, t->min_depth, counters);
+  if (t->traversal == MLOD_TRAVERSAL_WAVE)
+    k_project_wave<FORM, SEM, KT, CNT><<<(m + PBS - 1) / PBS, PBS, 0, s>>>(
+        t->view(), kp, t->qs.q, t->qs.qn, t->qs.perm, m, limit, nIter, stepMax, t->cutoff, t->min_depth, counters);
+  else
+    k_project<FORM, SEM, KT, CNT><<<(m + PBS - 1) / PBS, PBS, 0, s>>>(t->view(), kp, t->qs.q, t->qs.qn, t->qs.perm, m,
+                                                                       limit, nIter, stepMax, t->cutoff, t->min_depth,
+                                                                       counters);
   return MLOD_OK;
 }
 

@@ -11,7 +11,7 @@ def gpu_tree(points, normals, **kw):
 
 
 def gpu_project(tree, queries, kernel, cutoff=2.0, min_depth=0, semantics="ref_cpu", precision="fp32", n_iter=1,
-                mode="mlod", step_max=-1.0, knn=20, ball_radius=0.1):
+                mode="mlod", step_max=-1.0, knn=20, ball_radius=0.1, traversal="lane"):
     K = A.Kernel(**kernel) if isinstance(kernel, dict) else kernel
     cls = A.GlobalAPSS if mode == "global" else A.Fastapss
     if mode == "knn":
@@ -26,6 +26,7 @@ def gpu_project(tree, queries, kernel, cutoff=2.0, min_depth=0, semantics="ref_c
     ap.setMinDepth(min_depth)
     ap.set_semantics(semantics)
     ap.set_precision(precision)
+    ap.set_traversal(traversal)
     q = np.ascontiguousarray(queries, np.float32)
     ap.copyPointsToGPU(len(q), q)
     op = np.zeros((len(q), 3), np.float32)
