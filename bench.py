@@ -64,13 +64,19 @@ def algorithmic_work(counts, m, kern):
     return float(flops), float(trans)
 
 
+def kd_tree(w):
+    """A k-d tree over the workload's points (built once per workload), for ball statistics."""
+    if "_kd" not in w:
+        from scipy.spatial import cKDTree
+        w["_kd"] = cKDTree(w["points"].astype(np.float64))
+    return w["_kd"]
+
+
 def ball_population(w, sample=4000):
     """Mean number of points in a query's ball (radius 3 sigma_0), from a seeded query sample."""
-    from scipy.spatial import cKDTree
     r = 3.0 * w["kernel"]["sigmas"][0] if w["ball_radius"] < 0 else w["ball_radius"]
     q = w["queries"][np.random.default_rng(1).choice(len(w["queries"]), min(sample, len(w["queries"])), replace=False)]
-    t = cKDTree(w["points"])
-    return float(np.mean([len(x) for x in t.query_ball_point(q, r)]))
+    return float(np.mean(kd_tree(w).query_ball_point(q.astype(np.float64), r, return_length=True)))
 
 
 def cpu_baseline(w, kern, cutoff, budget_s=15.0):
@@ -543,12 +549,10 @@ def main():
                 # population (start positions; exact counts from a k-d tree over the points)
                 out_i = np.where(~ok | (np.nan_to_num(e, nan=1.0) > 1e-6))[0]
                 if len(out_i):
-                    from scipy.spatial import cKDTree
-                    if "_kd" not in w:
-                        w["_kd"] = cKDTree(P.astype(np.float64))
                     rb = 3.0 * w["kernel"]["sigmas"][0] if w["ball_radius"] < 0 else w["ball_radius"]
-                    cnt = np.array([len(x) for x in w["_kd"].query_ball_point(Q[out_i].astype(np.float64), rb)])
-                    deg = cnt < 4
+                    # fewer than 4 points in the ball <=> the 4th nearest point lies beyond r
+                    d4, _ = kd_tree(w).query(Q[out_i].astype(np.float64), k=4, distance_upper_bound=rb)
+                    deg = ~np.isfinite(d4[:, 3])
                     keep = np.ones(len(e), bool)
                     keep[out_i[deg]] = False
                     e2 = e[keep & ok]
