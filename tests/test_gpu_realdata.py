@@ -41,8 +41,21 @@ def test_real_model_parity(oracle_built, name):
             bad = err > 1e-5
             if prec == "fp32":
                 # the plain form keeps float arithmetic in the fit, like the reference; where the
-                # fit is ill-conditioned its rounding and the reference's part by more than the gate
+                # fit is ill-conditioned its rounding and the reference's part by more than the gate.
+                # There both are measured against the FP64 oracle: the GPU's float result must be no
+                # farther from the FP64 answer than the reference's own float result, within 2x
                 assert bad.mean() <= 1e-3 and err.max() < 1e-4
+                if bad.any():
+                    o = oracle_built.OracleTree(Pn, g["normals"])
+                    k = oracle_built.make_kernel(type=K.type, sigma=K.sigma, exponent=K.exponent,
+                                                 constant_shift=K.constant_shift, sigmas=K.sigmas)
+                    par = oracle_built.make_params(cutoff=prm.cutoff, min_depth=prm.min_depth, semantics="ref_cpu")
+                    o64, _ = o.project(g["queries"][bad], k, par, n_iter=it, precision=1)
+                    e_gpu = np.linalg.norm(p[bad].astype(np.float64) - o64, axis=1) / diag
+                    e_ref = np.linalg.norm(g[f"p{it}"][bad] - o64, axis=1) / diag
+                    print(f"  {bad.sum()} fp32 queries beyond the gate: from FP64, GPU {np.round(e_gpu, 8).tolist()} "
+                          f"reference {np.round(e_ref, 8).tolist()}")
+                    assert (e_gpu <= np.maximum(2 * e_ref, 1e-5)).all()
             elif bad.any():
                 # the gate against the reference's own CPU path, except on queries whose fit the
                 # reference's float arithmetic rounds beyond it (the oracle reproduces those bits):
