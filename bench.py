@@ -380,7 +380,7 @@ def run_extra_configs(names, timeout_s=420):
             continue
         r = json.loads(lines[-1])
         keep = ("value", "unit", "ms_per_step", "steps", "config", "forms", "error_vs_fp64", "counters_per_step",
-                "roofline", "adaptive_octree")
+                "counters_later_steps", "roofline", "adaptive_octree")
         out[name] = {k: r.get(k) for k in keep}
         out[name]["wall_s"] = time.time() - t0
     return out
@@ -488,6 +488,7 @@ def main():
         forms = [f for f in forms if f != "fp64"] + ["fp64"]
     results = {}
     counts = None
+    counts_later = None
     for form in forms:
         ap_ = A.Fastapss(tree, K)
         ap_.setScalingProtectionSphere(w["cutoff"])
@@ -524,6 +525,13 @@ def main():
         wall = time.perf_counter() - wall0
         launch_ms = [ev[s].elapsed_ms(ev[s + 1]) for s in range(args.steps)]
         total_ms = ev[0].elapsed_ms(ev[-1])
+        if form == args.precision and mode == "mlod":
+            # the same counters on the positions the timed steps reached: the walks of the steps after
+            # the first (queries on the surface) are longer and more alike than those of the first
+            # step from the offset positions; the counter pass leaves the positions as they are
+            counts_later = ap_.count_visits(stream=stream.ptr)
+            counts_later["lane_efficiency"] = counts_later["iterations"] / max(counts_later["lane_slots"], 1)
+            log(f"counters (a step after the timed steps): {counts_later}")
         pts, nrm = ap_.download(m, stream=stream.ptr)
         kinds = ap_.fit_kinds(m, stream=stream.ptr) if hasattr(lib_handle(), "mlod_queries_fit_kinds") else None
         results[form] = dict(total_ms=total_ms, wall_s=wall, launch_ms=launch_ms, pts=pts, nrm=nrm, kinds=kinds,
@@ -569,7 +577,13 @@ def main():
     head = results[args.precision]
     mean_launch_s = float(np.mean(head["launch_ms"])) / 1e3
     if mode == "mlod":
+        # the first timed step walks from the uploaded positions, the others from projected ones:
+        # the work per launch is the step-weighted mean of the two counter passes
         flops, trans = algorithmic_work(counts, m, w["kernel"])
+        if counts_later is not None and args.steps > 1:
+            f2, t2 = algorithmic_work(counts_later, m, w["kernel"])
+            flops = (flops + (args.steps - 1) * f2) / args.steps
+            trans = (trans + (args.steps - 1) * t2) / args.steps
     else:
         nb = float(w["knn"]) if mode == "knn" else ball_population(w)
         counts = {"neighbours_per_query": nb}
@@ -635,6 +649,7 @@ def main():
         "error_vs_fp64": errs,
         "forms": {f: {"pts_per_s": r["pts_per_s"], "ms_per_step": r["total_ms"] / args.steps} for f, r in results.items()},
         "counters_per_step": counts,
+        "counters_later_steps": counts_later,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
