@@ -396,6 +396,8 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--precision", default="fp32c", choices=["fp32", "fp32c", "fp64"])
     ap.add_argument("--semantics", default="ref_gpu", choices=["ref_gpu", "ref_cpu"])
+    ap.add_argument("--traversal", default="lane", choices=["lane", "wave"],
+                    help="MLoD walk: one per query (default) or one per wavefront over the union of its queries'")
     ap.add_argument("--no-forms", action="store_true", help="skip the plain FP32 / FP64 comparison runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -495,6 +497,7 @@ def main():
         ap_.setMinDepth(0)
         ap_.set_semantics(args.semantics)
         ap_.set_precision(form)
+        ap_.set_traversal(args.traversal)
         ap_.copyPointsToGPU_device(dQ.ptr.value, m, stream=stream.ptr)
         if counts is None and mode == "mlod":
             counts = ap_.count_visits(stream=stream.ptr)
@@ -598,7 +601,7 @@ def main():
                 "compulsory_hbm_frac": comp_bytes / mean_launch_s / 1e9 / PEAK_HBM_GBS,
                 "mean_launch_ms": mean_launch_s * 1e3}
     if args.workload == "superquadric8m" and args.scale == 1.0 and args.precision == "fp32c" and \
-            args.semantics == "ref_gpu":
+            args.semantics == "ref_gpu" and args.traversal == "lane":
         roofline["traffic"], roofline["traffic_profile_current"] = traffic_from_profile("k_project", args.workload)
     adaptive = None
     if args.workload == "torus1m":
@@ -644,6 +647,7 @@ def main():
                    "points": n, "queries": m, "kernel": {0: "Gaussian", 1: "rational", 2: "multi-Gaussian S=%d" % len(w["kernel"].get("sigmas", []))}[
                        w["kernel"]["type"]],
                    "cutoff": w["cutoff"], "min_depth": 0, "semantics": args.semantics, "precision": args.precision,
+                   "traversal": args.traversal,
                    "octree_nodes": info["n_nodes"], "octree_depth": info["depth"], "build_ms": build_ms,
                    "parallelism": "single GPU"},
         "error_vs_fp64": errs,
