@@ -600,6 +600,9 @@ def main():
                 "compulsory_bytes_per_launch": comp_bytes,
                 "compulsory_hbm_frac": comp_bytes / mean_launch_s / 1e9 / PEAK_HBM_GBS,
                 "mean_launch_ms": mean_launch_s * 1e3}
+    if mode != "mlod":
+        roofline["note"] = ("algorithmic work counts the fit over the neighbours only; the neighbour search, "
+                            "which dominates, is not counted (see the kernel's counter profile)")
     if args.workload == "superquadric8m" and args.scale == 1.0 and args.precision == "fp32c" and \
             args.semantics == "ref_gpu" and args.traversal == "lane":
         roofline["traffic"], roofline["traffic_profile_current"] = traffic_from_profile("k_project", args.workload)
