@@ -273,7 +273,10 @@ struct TopK {
 };
 
 template <int FORM, int KT, int KMAX>
-__global__ __launch_bounds__(NBS) void k_knn(TreeView tv, const float* __restrict__ box, KParams kp,
+// The 32 sorted 64-bit keys take 64 VGPRs; the FP32 forms are held to three waves per SIMD (the
+// compensated form then spills ~64 B per lane, cheaper than the lost wave), the FP64 form keeps two.
+__global__ __launch_bounds__(NBS) __attribute__((amdgpu_waves_per_eu(FORM == 2 ? 1 : 3)))
+void k_knn(TreeView tv, const float* __restrict__ box, KParams kp,
                                              float4* __restrict__ q, float4* __restrict__ qn,
                                              const uint32_t* __restrict__ perm, uint32_t m, uint32_t limit, uint32_t nIter,
                                              uint32_t k) {
@@ -313,24 +316,27 @@ __global__ __launch_bounds__(NBS) void k_knn(TreeView tv, const float* __restric
           // children whose box reaches the bound, visited nearest box first, so that the
           // first leaves reached are the query's own and the bound tightens at once
           const uint32_t first = h.x, nc = hdr_count(h.y);
-          float d[8];
-#pragma unroll
-          for (uint32_t c = 0; c < 8; ++c) d[c] = c < nc ? box_d2(box, first + c, qf) : INFINITY;
-          uint32_t order = 0, cnt = 0;
+          // keys: the box distance's bits (non-negative floats order as integers) with the child
+          // slot in the low 3 bits, children beyond the bound last; an 8-input sorting network
+          // (19 compare-exchanges) orders them.  The visiting order only steers the search: the
+          // k nearest do not depend on it
+          uint32_t key[8], cnt = 0;
 #pragma unroll
           for (uint32_t c = 0; c < 8; ++c) {
-            const bool in = c < nc && d[c] <= bound * kSlack;
-            uint32_t rank = 0;
-#pragma unroll
-            for (uint32_t o = 0; o < 8; ++o) {
-              const bool oin = o < nc && d[o] <= bound * kSlack;
-              rank += (oin && (d[o] < d[c] || (d[o] == d[c] && o < c))) ? 1u : 0u;
-            }
-            if (in) {
-              order |= c << (4u * rank);
-              ++cnt;
-            }
+            const float d = c < nc ? box_d2(box, first + c, qf) : INFINITY;
+            const bool in = c < nc && d <= bound * kSlack;
+            key[c] = in ? (__float_as_uint(d) & ~7u) | c : 0xffffffffu;
+            cnt += in ? 1u : 0u;
           }
+          const auto cx = [&](int a, int b) {
+            const uint32_t lo = min(key[a], key[b]), hi = max(key[a], key[b]);
+            key[a] = lo, key[b] = hi;
+          };
+          cx(0, 2), cx(1, 3), cx(4, 6), cx(5, 7), cx(0, 4), cx(1, 5), cx(2, 6), cx(3, 7), cx(0, 1), cx(2, 3);
+          cx(4, 5), cx(6, 7), cx(2, 4), cx(3, 5), cx(1, 4), cx(3, 6), cx(1, 2), cx(3, 4), cx(5, 6);
+          uint32_t order = 0;
+#pragma unroll
+          for (uint32_t r = 0; r < 8; ++r) order |= (key[r] & 7u) << (4u * r);
           ++lvl;
           sFirst[lvl][tid] = first;
           sOrder[lvl][tid] = order;
